@@ -1,0 +1,1348 @@
+// engine.hip -- gfx950 kernels of one BUSseq Gibbs sweep and the host driver
+// that allocates the chains, captures the sweep into a hipGraph and replays it.
+//
+// Sweep (one hipGraph, thirteen launches, chains on blockIdx.z):
+//   k_prep       alpha/gamma proposals, zero the per-sweep counters
+//   k_w          W_i: categorical full conditional, one wave per cell     [row pass]
+//   k_plan       pi_b Dirichlet draw, stable sort of cells by (b, W), tiles
+//   k_xz_alpha   X/Z imputation, gamma histogram, alpha partial sums     [tile pass]
+//   k_alpha_dec  alpha accept, L draw, beta/nu/phi proposals
+//   k_gamma_ptau gamma0/gamma1 accept per batch; p, tau0 draw
+//   k_beta_nu    beta and nu partial sums (nu under both beta outcomes)  [tile pass]
+//   k_bn_dec     beta then nu accept
+//   k_delta      delta proposal, row sum and accept, one wave per cell   [row pass]
+//   k_phi        phi partial sums with the new delta                     [tile pass]
+//   k_phi_dec    phi accept
+//   k_record     posterior accumulators and traces after burn-in
+//   k_tick       iteration counter
+//
+// Row passes: lane l of the wave owning cell i sums genes g = l (mod 64) in
+// double precision; the 64 lane sums are added exactly in fixed point.
+// Tile passes: a workgroup owns one tile (<= TR cells of one (batch, subtype)
+// group, in stable sorted order) and 256 genes; each lane sums its gene over
+// the tile in double precision and stores the exact fixed-point value.  The
+// decision kernels add tile values exactly.  The host oracle uses the same
+// grouping, so the sampler reproduces it bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/busseq.h"
+#include "../../include/busseq/model.h"
+#include "engine.h"
+
+#define H BQ_HIST
+
+void bq_set_error(const std::string& m);
+
+// ---------------------------------------------------------------------------
+// precision-overloaded element terms
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float tmu(float a, float b, float c, float d) { return bq_mu_f32(a, b, c, d); }
+__device__ __forceinline__ double tmu(double a, double b, double c, double d) { return bq_mu_f64(a, b, c, d); }
+__device__ __forceinline__ float tratio(float m, float p) { return bq_ratio_f32(m, p); }
+__device__ __forceinline__ double tratio(double m, double p) { return bq_ratio_f64(m, p); }
+__device__ __forceinline__ float tlog1p(float x) { return bq_log1pf(x); }
+__device__ __forceinline__ double tlog1p(double x) { return bq_log1p(x); }
+__device__ __forceinline__ float ttw(float x, float p, float r, float b, float e) { return bq_tw_f32(x, p, r, b, e); }
+__device__ __forceinline__ double ttw(double x, double p, double r, double b, double e) { return bq_tw_f64(x, p, r, b, e); }
+__device__ __forceinline__ float ttmh(float x, float p, float r, float d, float c) { return bq_tmh_f32(x, p, r, d, c); }
+__device__ __forceinline__ double ttmh(double x, double p, double r, double d, double c) { return bq_tmh_f64(x, p, r, d, c); }
+__device__ __forceinline__ float ttmh_l(float x, float p, float d, float l) { return bq_tmh_l_f32(x, p, d, l); }
+__device__ __forceinline__ double ttmh_l(double x, double p, double d, double l) { return bq_tmh_l_f64(x, p, d, l); }
+__device__ __forceinline__ float ttmh2(float x, float p, float r, float d, float c, float l) { return bq_tmh2_f32(x, p, r, d, c, l); }
+__device__ __forceinline__ double ttmh2(double x, double p, double r, double d, double c, double l) { return bq_tmh2_f64(x, p, r, d, c, l); }
+__device__ __forceinline__ float ttphi(int xi, float x, float ph, float pp, float mu, float d, float dop, float ip, float lr, float dl)
+{ return bq_tphi_f32(xi, x, ph, pp, mu, d, dop, ip, lr, dl); }
+__device__ __forceinline__ double ttphi(int xi, double x, double ph, double pp, double mu, double d, double dop, double ip, double lr, double dl)
+{ return bq_tphi_f64(xi, x, ph, pp, mu, d, dop, ip, lr, dl); }
+
+// exact wave-wide sum of a fixed-point value (wrapping uint64 adds are associative)
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        uint32_t lo = __shfl_xor((uint32_t)v, m, 64);
+        uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+        v += ((uint64_t)hi << 32) | (uint64_t)lo;
+    }
+    return v;
+}
+
+// exact block-wide sum (256 threads = 4 waves); result valid in every thread
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh)
+{
+    v = wave_sum_u64(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double ph_uniform(const ChainDev& c, uint32_t a, uint32_t b, int t, int phase)
+{
+    bq_rng r;
+    bq_rng_init(&r, c.seed, c.key, a, b, (uint32_t)t, (uint32_t)phase);
+    return bq_uniform(&r);
+}
+
+// ---------------------------------------------------------------------------
+// k_prep: alpha and gamma proposals (cu:514-522, 602-606), counters to zero
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int t = *c.iter;
+    if (j < c.G) {
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)j, 0u, (uint32_t)t, BQ_PH_ALPHA_P);
+        double a = c.alpha[j];
+        double ap = a + BQ_RW_SD * bq_normal(&r);
+        c.alpha_p[j] = ap;
+        double dl = ap - a;
+        ((T*)c.da)[j] = (T)dl;
+        ((T*)c.ca)[j] = (T)bq_expm1(dl);
+    }
+    if (j < c.B) {
+        const int B = c.B;
+        bq_rng r0;
+        bq_rng_init(&r0, c.seed, c.key, (uint32_t)j, 0u, (uint32_t)t, BQ_PH_GAMMA0_P);
+        c.gamma_p[j] = c.gamma[j] + BQ_RW_SD * bq_normal(&r0);
+        bq_rng r1;
+        bq_rng_init(&r1, c.seed, c.key, (uint32_t)j, 0u, (uint32_t)t, BQ_PH_GAMMA1_P);
+        c.gamma_p[B + j] = -bq_rgamma(&r1, -10.0 * c.gamma[B + j], 0.1);
+        for (int q = 0; q < 5; ++q) c.gstat[j * 5 + q] = 0;
+    }
+    if (j < c.B * H) c.hist_d[j] = 0u;
+    if (j < c.B * c.K) c.cnt[j] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// k_w: subtype draw from the categorical full conditional (replaces the MH
+// step of cu:820-854); logit_k = log pi_bk + sum_g [x beta_k - (phi+x) log1p(r0 expm1 beta_k)]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= c.N) return;
+    const int t = *c.iter;
+    const int G = c.G, K = c.K, b = c.batch[i];
+    const T* ea = (const T*)c.ea;
+    const T* en = (const T*)c.en + (size_t)b * G;
+    const T* ph = (const T*)c.ph + (size_t)b * G;
+    const T* bt = (const T*)c.bt;
+    const T* em = (const T*)c.em;
+    const T edi = ((const T*)c.ed)[i];
+    const int32_t* row = c.S + (size_t)i * c.Gp;
+
+    double part[BQ_KMAX];
+#pragma unroll
+    for (int k = 0; k < BQ_KMAX; ++k) part[k] = 0.0;
+    for (int g = lane; g < G; g += 64) {
+        int32_t s = row[g];
+        int xi = s < 0 ? ~s : s;
+        T x = (T)xi;
+        T phi = ph[g];
+        T mu0 = tmu(ea[g], (T)1, en[g], edi);
+        T r0 = tratio(mu0, phi);
+#pragma unroll
+        for (int k = 1; k < BQ_KMAX; ++k)
+            if (k < K) part[k] += (double)ttw(x, phi, r0, bt[(size_t)k * G + g], em[(size_t)k * G + g]);
+    }
+    double lp[BQ_KMAX];
+    double mx = -BQ_INF;
+#pragma unroll
+    for (int k = 0; k < BQ_KMAX; ++k) {
+        if (k < K) {
+            uint64_t a = 0;
+            if (k > 0) a = wave_sum_u64(bq_fix(part[k]));
+            double lk = bq_unfix(a);
+            lp[k] = lk + bq_log(c.pi[b * K + k]);
+            if (lp[k] > mx) mx = lp[k];
+        }
+    }
+    double tot = 0.0;
+#pragma unroll
+    for (int k = 0; k < BQ_KMAX; ++k)
+        if (k < K) { lp[k] = bq_exp(lp[k] - mx); tot += lp[k]; }
+    double tgt = ph_uniform(c, (uint32_t)i, 0u, t, BQ_PH_W) * tot, cum = 0.0;
+    int kk = K - 1, found = 0;
+#pragma unroll
+    for (int k = 0; k < BQ_KMAX; ++k)
+        if (k < K && !found) { cum += lp[k]; if (tgt <= cum) { kk = k; found = 1; } }
+    if (lane == 0) {
+        c.W[i] = kk;
+        atomicAdd(&c.cnt[b * K + kk], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_plan: pi_b ~ Dirichlet(n_b. + xi) (cu:856-890, normalisation race fixed),
+// stable partition of batch b's cells by subtype, tile list.  One block per batch.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int b = blockIdx.x;
+    if (b >= c.B) return;
+    const int K = c.K, B = c.B, TR = c.TR, tid = threadIdx.x;
+    const int t = *c.iter;
+    __shared__ int s_toff[BQ_BMAX * BQ_KMAX + 1];
+    __shared__ int s_cnt[BQ_BMAX * BQ_KMAX];
+    __shared__ int s_pos[256 * BQ_KMAX];
+    for (int j = tid; j < B * K; j += 256) s_cnt[j] = c.cnt[j];
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int j = 0; j < B * K; ++j) { s_toff[j] = acc; acc += (s_cnt[j] + TR - 1) / TR; }
+        s_toff[B * K] = acc;
+    }
+    __syncthreads();
+    if (b == 0) {
+        for (int j = tid; j <= B * K; j += 256) c.tile_off[j] = s_toff[j];
+        if (tid == 0) *c.ntiles = s_toff[B * K];
+    }
+    // tiles of this batch's groups
+    const int b0 = c.boff[b], nb = c.boff[b + 1] - b0;
+    int gstart = b0;
+    for (int k = 0; k < K; ++k) {
+        int n = s_cnt[b * K + k], t0 = s_toff[b * K + k];
+        for (int j = tid; j * TR < n; j += 256) {
+            int q = t0 + j;
+            c.tiles[4 * q] = b;
+            c.tiles[4 * q + 1] = k;
+            c.tiles[4 * q + 2] = gstart + j * TR;
+            c.tiles[4 * q + 3] = n - j * TR < TR ? n - j * TR : TR;
+        }
+        gstart += n;
+    }
+    // pi_b
+    if (tid == 0) {
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)b, 0u, (uint32_t)t, BQ_PH_PI);
+        double gk[BQ_KMAX], tot = 0.0;
+        for (int k = 0; k < K; ++k) { gk[k] = bq_rgamma(&r, s_cnt[b * K + k] + BQ_XI_PI, 1.0); tot += gk[k]; }
+        for (int k = 0; k < K; ++k) c.pi[b * K + k] = gk[k] / tot;
+    }
+    // stable partition: thread tid owns cells [lo, hi) of the batch
+    const int chunk = (nb + 255) / 256;
+    const int lo = b0 + tid * chunk;
+    const int hi = lo + chunk < b0 + nb ? lo + chunk : b0 + nb;
+    int my[BQ_KMAX];
+#pragma unroll
+    for (int k = 0; k < BQ_KMAX; ++k) my[k] = 0;
+    for (int i = lo; i < hi; ++i) {
+        int w = c.W[i];
+#pragma unroll
+        for (int k = 0; k < BQ_KMAX; ++k) my[k] += (w == k);
+    }
+#pragma unroll
+    for (int k = 0; k < BQ_KMAX; ++k) s_pos[k * 256 + tid] = my[k];
+    __syncthreads();
+    if (tid < K) {
+        int acc = b0;
+        for (int k = 0; k < tid; ++k) acc += s_cnt[b * K + k];
+        for (int q = 0; q < 256; ++q) { int v = s_pos[tid * 256 + q]; s_pos[tid * 256 + q] = acc; acc += v; }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BQ_KMAX; ++k) my[k] = s_pos[k * 256 + tid];
+    for (int i = lo; i < hi; ++i) {
+        int w = c.W[i];
+        int pos = 0;
+#pragma unroll
+        for (int k = 0; k < BQ_KMAX; ++k)
+            if (w == k) { pos = my[k]; my[k] += 1; }
+        c.perm[pos] = i;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_xz_alpha: X/Z imputation (cu:484-512), dynamic gamma histogram, alpha
+// partial sums (cu:608-623).  Block = 256 genes x one tile.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_xz_alpha(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int tile = blockIdx.y;
+    if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
+    __shared__ uint32_t hist[H];
+    __shared__ uint64_t s_red[4];
+    const int tid = threadIdx.x;
+    for (int x = tid; x < H; x += 256) hist[x] = 0u;
+    __syncthreads();
+    const int G = c.G, B = c.B, t = *c.iter;
+    const int b = c.tiles[4 * tile], k = c.tiles[4 * tile + 1];
+    const int st = c.tiles[4 * tile + 2], nr = c.tiles[4 * tile + 3];
+    const int g = blockIdx.x * 256 + tid;
+    const double g0 = c.gamma[b], g1 = c.gamma[B + b];
+    const double g0p = c.gamma_p[b], g1p = c.gamma_p[B + b];
+    const double qz = 1.0 / (1.0 + bq_exp(g0));
+    uint64_t my_n1 = 0, my_sx1 = 0;
+    if (g < G) {
+        const size_t bg = (size_t)b * G + g;
+        const T eag = ((const T*)c.ea)[g], ebg = ((const T*)c.eb)[(size_t)k * G + g];
+        const T eng = ((const T*)c.en)[bg], phg = ((const T*)c.ph)[bg];
+        const T dag = ((const T*)c.da)[g], cag = ((const T*)c.ca)[g];
+        const T* ed = (const T*)c.ed;
+        const double phid = c.phi[bg];
+        double acc = 0.0;
+        for (int rr = 0; rr < nr; ++rr) {
+            const int i = c.perm[st + rr];
+            const size_t e = (size_t)i * c.Gp + g;
+            const int32_t s = c.S[e];
+            const T mu = tmu(eag, ebg, eng, ed[i]);
+            int xi;
+            if (s <= 0) {
+                bq_rng r;
+                bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ);
+                int xo = s < 0 ? ~s : 0;
+                int z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
+                if (z) {
+                    int xp = bq_rnbinom(&r, (double)mu, phid);
+                    double lr = bq_neg_logit_sp(g0, g1, (double)xo) - bq_neg_logit_sp(g0, g1, (double)xp);
+                    xi = bq_log(bq_uniform(&r)) <= lr ? xp : xo;
+                    c.S[e] = ~xi;
+                    my_n1 += 1;
+                    my_sx1 += (uint64_t)xi;
+                } else {
+                    xi = 0;
+                    c.S[e] = 0;
+                }
+                if (xi < H) atomicAdd(&hist[xi], 1u);
+            } else {
+                xi = s;
+            }
+            if (xi >= H) {
+                double xd = (double)xi;
+                double a0 = bq_sp_lin(g0, g1, xd), a1 = bq_sp_lin(g0p, g1, xd);
+                double a2 = bq_sp_lin(g0, g1p, xd), a3 = bq_sp_lin(g0p, g1p, xd);
+                atomicAdd((unsigned long long*)&c.gstat[b * 5 + 2], (unsigned long long)bq_fix(a0 - a1));
+                atomicAdd((unsigned long long*)&c.gstat[b * 5 + 3], (unsigned long long)bq_fix(a0 - a2));
+                atomicAdd((unsigned long long*)&c.gstat[b * 5 + 4], (unsigned long long)bq_fix(a1 - a3));
+            }
+            const T rt = tratio(mu, phg);
+            acc += (double)ttmh((T)xi, phg, rt, dag, cag);
+        }
+        c.part[((size_t)tile * 3 + 0) * c.Gp + g] = bq_fix(acc);
+    }
+    uint64_t n1 = block_sum_u64(my_n1, s_red);
+    uint64_t sx1 = block_sum_u64(my_sx1, s_red);
+    for (int x = tid; x < H; x += 256) {
+        uint32_t v = hist[x];
+        if (v) atomicAdd(&c.hist_d[b * H + x], v);
+    }
+    if (tid == 0) {
+        if (n1) atomicAdd((unsigned long long*)&c.gstat[b * 5 + 0], (unsigned long long)n1);
+        if (sx1) atomicAdd((unsigned long long*)&c.gstat[b * 5 + 1], (unsigned long long)sx1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_alpha_dec: alpha accept (cu:625-632); L Gibbs (cu:634-648) and its sums for
+// p/tau0; beta (cu:670-674), nu (cu:709-713) and phi (cu:780-787) proposals.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= c.G) return;
+    const int G = c.G, K = c.K, B = c.B, t = *c.iter, nt = *c.ntiles;
+    const size_t Gp = c.Gp;
+    uint64_t a = 0;
+    for (int tt = 0; tt < nt; ++tt) a += c.part[((size_t)tt * 3) * Gp + g];
+    double sum = bq_unfix(a);
+    double al = c.alpha[g], ap = c.alpha_p[g];
+    double lr = sum + bq_lr_normal(al, ap, c.mu_alpha[g], BQ_SIGMA2_ALPHA);
+    if (bq_log(ph_uniform(c, (uint32_t)g, 0u, t, BQ_PH_ALPHA_A)) <= lr) {
+        c.alpha[g] = ap;
+        ((T*)c.ea)[g] = (T)bq_exp(ap);
+    }
+    // L with the current p, tau0
+    const double p = c.ptau[0], tau0 = c.ptau[1];
+    int ls = 0;
+    uint64_t bs = 0;
+    for (int k = 1; k < K; ++k) {
+        size_t j = (size_t)k * G + g;
+        double be = c.beta[j];
+        int L = bq_bern_logodds(ph_uniform(c, (uint32_t)g, (uint32_t)k, t, BQ_PH_L), bq_logodds_L(p, tau0, be));
+        c.L[j] = L;
+        if (L) ls += 1; else bs += bq_fix(be * be);
+    }
+    c.lsum[g] = ls;
+    c.bsq[g] = bs;
+    // beta, nu proposals and their T constants
+    for (int k = 0; k < K; ++k) {
+        size_t j = (size_t)k * G + g;
+        double be = c.beta[j], bp = be;
+        if (k > 0) {
+            bq_rng r;
+            bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)k, (uint32_t)t, BQ_PH_BETA_P);
+            bp = be + BQ_RW_SD * bq_normal(&r);
+        }
+        c.beta_p[j] = bp;
+        double dl = bp - be;
+        ((T*)c.db)[j] = (T)dl;
+        ((T*)c.cb)[j] = (T)bq_expm1(dl);
+    }
+    for (int b = 0; b < B; ++b) {
+        size_t j = (size_t)b * G + g;
+        double nv = c.nu[j], np = nv;
+        if (b > 0) {
+            bq_rng r;
+            bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)b, (uint32_t)t, BQ_PH_NU_P);
+            np = nv + BQ_RW_SD * bq_normal(&r);
+        }
+        c.nu_p[j] = np;
+        double dl = np - nv;
+        ((T*)c.dn)[j] = (T)dl;
+        ((T*)c.cn)[j] = (T)bq_expm1(dl);
+    }
+    // phi proposals and the per-(b,g) constants of the phi terms
+    const size_t BG = (size_t)B * G;
+    T* pc = (T*)c.pc;
+    for (int b = 0; b < B; ++b) {
+        size_t j = (size_t)b * G + g;
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)b, (uint32_t)t, BQ_PH_PHI_P);
+        double f = c.phi[j];
+        double fp = bq_rgamma(&r, f, 1.0);
+        c.phi_p[j] = fp;
+        if (!(fp > 0.0)) fp = f;
+        double d = fp - f;
+        pc[j] = (T)fp;
+        pc[BG + j] = (T)d;
+        pc[2 * BG + j] = (T)(d / f);
+        pc[3 * BG + j] = (T)(1.0 / fp);
+        pc[4 * BG + j] = (T)bq_log1p(d / f);
+        pc[5 * BG + j] = (T)bq_lgamma_diff64(f, d);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_gamma_ptau: gamma0 then gamma1 accept per batch from the X histogram
+// (cu:524-600); the last block draws p and tau0 (cu:650-668)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    __shared__ uint64_t s_red[4];
+    __shared__ double s_g0;
+    __shared__ int s_acc0;
+    const int tid = threadIdx.x, B = c.B, t = *c.iter;
+    if (blockIdx.x == gridDim.x - 1) {
+        if (t < c.n_unchanged) return;
+        const int G = c.G, K = c.K;
+        uint64_t cl = 0, sb = 0;
+        for (int g = tid; g < G; g += 256) { cl += (uint64_t)c.lsum[g]; sb += c.bsq[g]; }
+        cl = block_sum_u64(cl, s_red);
+        sb = block_sum_u64(sb, s_red);
+        if (tid == 0) {
+            double n = (double)G * (K - 1);
+            double cnt = (double)(int64_t)cl;
+            bq_rng r;
+            bq_rng_init(&r, c.seed, c.key, 0u, 0u, (uint32_t)t, BQ_PH_PTAU);
+            c.ptau[0] = bq_rbeta(&r, cnt + BQ_A_P, n - cnt + BQ_B_P);
+            double a = BQ_A_TAU0 + (n - cnt) / 2.0;
+            double bb = BQ_B_TAU0 + bq_unfix(sb) / 2.0;
+            c.ptau[1] = 1.0 / bq_rgamma(&r, a, 1.0 / bb);
+        }
+        return;
+    }
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const int64_t* hs = c.hist_s + (size_t)b * H;
+    const uint32_t* hd = c.hist_d + (size_t)b * H;
+    const double g0 = c.gamma[b], g1 = c.gamma[B + b];
+    const double g0p = c.gamma_p[b], g1p = c.gamma_p[B + b];
+    uint64_t a0 = 0;
+    for (int x = tid; x < H; x += 256) {
+        int64_t n = hs[x] + (int64_t)hd[x];
+        if (n) a0 += bq_fix((double)n * (bq_sp_lin(g0, g1, (double)x) - bq_sp_lin(g0p, g1, (double)x)));
+    }
+    a0 = block_sum_u64(a0, s_red);
+    if (tid == 0) {
+        const uint64_t* gs = c.gstat + b * 5;
+        a0 += bq_fix((g0p - g0) * (double)(int64_t)gs[0]);
+        a0 += gs[2];
+        double s0 = bq_unfix(a0);
+        double lr0 = s0 + bq_lr_normal(g0, g0p, 0.0, BQ_SIGMA2_GAMMA0);
+        int acc0 = bq_log(ph_uniform(c, (uint32_t)b, 0u, t, BQ_PH_GAMMA0_A)) <= lr0;
+        if (acc0) c.gamma[b] = g0p;
+        s_g0 = acc0 ? g0p : g0;
+        s_acc0 = acc0;
+    }
+    __syncthreads();
+    const double g0n = s_g0;
+    uint64_t a1 = 0;
+    for (int x = tid; x < H; x += 256) {
+        int64_t n = hs[x] + (int64_t)hd[x];
+        if (n) a1 += bq_fix((double)n * (bq_sp_lin(g0n, g1, (double)x) - bq_sp_lin(g0n, g1p, (double)x)));
+    }
+    a1 = block_sum_u64(a1, s_red);
+    if (tid == 0) {
+        const uint64_t* gs = c.gstat + b * 5;
+        a1 += bq_fix((g1p - g1) * (double)(int64_t)gs[1]);
+        a1 += s_acc0 ? gs[4] : gs[3];
+        double s1 = bq_unfix(a1);
+        double u1 = ph_uniform(c, (uint32_t)b, 0u, t, BQ_PH_GAMMA1_A);
+        if (g1p < 0.0) {
+            double lr1 = s1 + bq_lr_gamma1(-g1, -g1p);
+            if (bq_log(u1) <= lr1) c.gamma[B + b] = g1p;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_beta_nu: beta (cu:676-694) and nu (cu:715-728) partial sums; the nu sum
+// is kept for both outcomes of the beta step of the cell's subtype
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int tile = blockIdx.y;
+    if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= c.G) return;
+    const int G = c.G;
+    const int b = c.tiles[4 * tile], k = c.tiles[4 * tile + 1];
+    const int st = c.tiles[4 * tile + 2], nr = c.tiles[4 * tile + 3];
+    const size_t bg = (size_t)b * G + g, kg = (size_t)k * G + g;
+    const T eag = ((const T*)c.ea)[g], ebg = ((const T*)c.eb)[kg];
+    const T eng = ((const T*)c.en)[bg], phg = ((const T*)c.ph)[bg];
+    const T dbg = ((const T*)c.db)[kg], cbg = ((const T*)c.cb)[kg];
+    const T dng = ((const T*)c.dn)[bg], cng = ((const T*)c.cn)[bg];
+    const T cbn = (cbg + cng) + cbg * cng;
+    const T* ed = (const T*)c.ed;
+    double pb = 0.0, pn0 = 0.0, pn1 = 0.0;
+    for (int rr = 0; rr < nr; ++rr) {
+        const int i = c.perm[st + rr];
+        const int32_t s = c.S[(size_t)i * c.Gp + g];
+        const int xi = s < 0 ? ~s : s;
+        const T x = (T)xi;
+        const T mu = tmu(eag, ebg, eng, ed[i]);
+        const T rt = tratio(mu, phg);
+        T lb = (T)0;
+        if (k >= 1) {
+            lb = tlog1p(rt * cbg);
+            pb += (double)ttmh_l(x, phg, dbg, lb);
+        }
+        if (b >= 1) {
+            pn0 += (double)ttmh(x, phg, rt, dng, cng);
+            if (k >= 1) pn1 += (double)ttmh2(x, phg, rt, dng, cbn, lb);
+        }
+    }
+    uint64_t* pt = c.part + (size_t)tile * 3 * c.Gp;
+    pt[g] = bq_fix(pb);
+    pt[c.Gp + g] = bq_fix(pn0);
+    pt[2 * (size_t)c.Gp + g] = bq_fix(pn1);
+}
+
+// ---------------------------------------------------------------------------
+// k_bn_dec: beta accept (cu:696-707) for all k, then nu accept (cu:730-737)
+// with the nu sums of the accepted beta branch
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_bn_dec(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= c.G) return;
+    const int G = c.G, K = c.K, B = c.B, t = *c.iter;
+    const size_t Gp = c.Gp;
+    const int* toff = c.tile_off;
+    uint64_t ab[BQ_KMAX];
+#pragma unroll
+    for (int k = 0; k < BQ_KMAX; ++k) ab[k] = 0;
+    for (int b = 0; b < B; ++b) {
+#pragma unroll
+        for (int k = 1; k < BQ_KMAX; ++k) {
+            if (k < K) {
+                int e = toff[b * K + k + 1];
+                for (int tt = toff[b * K + k]; tt < e; ++tt) ab[k] += c.part[((size_t)tt * 3) * Gp + g];
+            }
+        }
+    }
+    const double tau0 = c.ptau[1];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 1; k < BQ_KMAX; ++k) {
+        if (k < K) {
+            size_t j = (size_t)k * G + g;
+            double sum = bq_unfix(ab[k]);
+            double be = c.beta[j], bp = c.beta_p[j];
+            double lr = sum + bq_lr_beta(be, bp, c.L[j], tau0);
+            if (bq_log(ph_uniform(c, (uint32_t)g, (uint32_t)k, t, BQ_PH_BETA_A)) <= lr) {
+                c.beta[j] = bp;
+                ((T*)c.eb)[j] = (T)bq_exp(bp);
+                ((T*)c.em)[j] = (T)bq_expm1(bp);
+                ((T*)c.bt)[j] = (T)bp;
+                mask |= 1u << k;
+            }
+        }
+    }
+    for (int b = 1; b < B; ++b) {
+        uint64_t a = 0;
+        for (int k = 0; k < K; ++k) {
+            int slot = (k >= 1 && ((mask >> k) & 1u)) ? 2 : 1;
+            int e = toff[b * K + k + 1];
+            for (int tt = toff[b * K + k]; tt < e; ++tt) a += c.part[((size_t)tt * 3 + slot) * Gp + g];
+        }
+        size_t j = (size_t)b * G + g;
+        double sum = bq_unfix(a);
+        double nv = c.nu[j], np = c.nu_p[j];
+        double lr = sum + bq_lr_normal(nv, np, c.mu_nu[b], BQ_SIGMA2_NU);
+        if (bq_log(ph_uniform(c, (uint32_t)g, (uint32_t)b, t, BQ_PH_NU_A)) <= lr) {
+            c.nu[j] = np;
+            ((T*)c.en)[j] = (T)bq_exp(np);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_delta: delta proposal (cu:739-744), row sum (cu:746-761), accept
+// (cu:763-778); the first cell of each batch keeps delta = 0
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= c.N) return;
+    const int b = c.batch[i];
+    if (i == c.boff[b]) return;
+    const int t = *c.iter;
+    const int G = c.G, k = c.W[i];
+    const T* ea = (const T*)c.ea;
+    const T* eb = (const T*)c.eb + (size_t)k * G;
+    const T* en = (const T*)c.en + (size_t)b * G;
+    const T* ph = (const T*)c.ph + (size_t)b * G;
+    T* ed = (T*)c.ed;
+    const T edi = ed[i];
+    const double del = c.delta[i];
+    bq_rng r;
+    bq_rng_init(&r, c.seed, c.key, (uint32_t)i, 0u, (uint32_t)t, BQ_PH_DELTA_P);
+    const double dp = del + BQ_RW_SD * bq_normal(&r);
+    const double dl = dp - del;
+    const T dd = (T)dl, cd = (T)bq_expm1(dl);
+    const int32_t* row = c.S + (size_t)i * c.Gp;
+    double part = 0.0;
+    for (int g = lane; g < G; g += 64) {
+        int32_t s = row[g];
+        int xi = s < 0 ? ~s : s;
+        T phi = ph[g];
+        T mu = tmu(ea[g], eb[g], en[g], edi);
+        T rt = tratio(mu, phi);
+        part += (double)ttmh((T)xi, phi, rt, dd, cd);
+    }
+    uint64_t a = wave_sum_u64(bq_fix(part));
+    double sum = bq_unfix(a);
+    double lr = sum + bq_lr_normal(del, dp, c.mu_delta[i], BQ_SIGMA2_DELTA);
+    if (bq_log(ph_uniform(c, (uint32_t)i, 0u, t, BQ_PH_DELTA_A)) <= lr) {
+        if (lane == 0) {
+            c.delta[i] = dp;
+            ed[i] = (T)bq_exp(dp);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_phi: phi partial sums (cu:789-804) with the new delta
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_phi(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int tile = blockIdx.y;
+    if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= c.G) return;
+    const int G = c.G;
+    const int b = c.tiles[4 * tile], k = c.tiles[4 * tile + 1];
+    const int st = c.tiles[4 * tile + 2], nr = c.tiles[4 * tile + 3];
+    const size_t bg = (size_t)b * G + g, BG = (size_t)c.B * G;
+    const T* pc = (const T*)c.pc;
+    const T eag = ((const T*)c.ea)[g], ebg = ((const T*)c.eb)[(size_t)k * G + g];
+    const T eng = ((const T*)c.en)[bg], phg = ((const T*)c.ph)[bg];
+    const T fp = pc[bg], d = pc[BG + bg], dop = pc[2 * BG + bg], ip = pc[3 * BG + bg];
+    const T lrat = pc[4 * BG + bg], dlg0 = pc[5 * BG + bg];
+    const T* ed = (const T*)c.ed;
+    double acc = 0.0;
+    for (int rr = 0; rr < nr; ++rr) {
+        const int i = c.perm[st + rr];
+        const int32_t s = c.S[(size_t)i * c.Gp + g];
+        const int xi = s < 0 ? ~s : s;
+        const T mu = tmu(eag, ebg, eng, ed[i]);
+        acc += (double)ttphi(xi, (T)xi, phg, fp, mu, d, dop, ip, lrat, dlg0);
+    }
+    c.part[((size_t)tile * 3) * c.Gp + g] = bq_fix(acc);
+}
+
+// ---------------------------------------------------------------------------
+// k_phi_dec: phi accept (cu:806-818)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_phi_dec(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= c.G) return;
+    const int G = c.G, K = c.K, B = c.B, t = *c.iter;
+    const size_t Gp = c.Gp;
+    for (int b = 0; b < B; ++b) {
+        uint64_t a = 0;
+        int e = c.tile_off[(b + 1) * K];
+        for (int tt = c.tile_off[b * K]; tt < e; ++tt) a += c.part[((size_t)tt * 3) * Gp + g];
+        size_t j = (size_t)b * G + g;
+        double sum = bq_unfix(a);
+        double f = c.phi[j], fp = c.phi_p[j];
+        double u = ph_uniform(c, (uint32_t)g, (uint32_t)b, t, BQ_PH_PHI_A);
+        if (!(fp > 0.0)) continue;
+        double lr = sum + bq_lr_phi(f, fp);
+        if (bq_log(u) <= lr) {
+            c.phi[j] = fp;
+            ((T*)c.ph)[j] = (T)fp;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_record: posterior accumulators and traces (cu:892-936, 1547-1554)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_record(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int t = *c.iter;
+    if (t < c.n_burnin) return;
+    const int r = t - c.n_burnin;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int G = c.G, K = c.K, B = c.B, N = c.N, P = c.n_pres;
+    const bool tr = c.traces && r < P;
+    if (j == 0) *c.n_rec += 1;
+    if (j < 2 * B) {
+        c.s_gamma[j] += c.gamma[j];
+        if (tr) c.tr_gamma[(size_t)j * P + r] = c.gamma[j];
+    }
+    if (j < 2) {
+        c.s_ptau[j] += c.ptau[j];
+        if (tr) c.tr_ptau[(size_t)j * P + r] = c.ptau[j];
+    }
+    if (j < G) {
+        c.s_alpha[j] += c.alpha[j];
+        if (tr) c.tr_alpha[(size_t)j * P + r] = c.alpha[j];
+    }
+    if (j < K * G) {
+        c.s_beta[j] += c.beta[j];
+        c.c_L[j] += c.L[j];
+        if (tr && j >= G) {
+            c.tr_beta[(size_t)(j - G) * P + r] = c.beta[j];
+            c.tr_L[(size_t)(j - G) * P + r] = c.L[j];
+        }
+    }
+    if (j < B * G) {
+        c.s_nu[j] += c.nu[j];
+        c.s_phi[j] += c.phi[j];
+        if (tr) {
+            if (j >= G) c.tr_nu[(size_t)(j - G) * P + r] = c.nu[j];
+            c.tr_phi[(size_t)j * P + r] = c.phi[j];
+        }
+    }
+    if (j < N) {
+        c.s_delta[j] += c.delta[j];
+        c.c_W[(size_t)j * K + c.W[j]] += 1;
+        if (tr) {
+            c.tr_delta[(size_t)j * P + r] = c.delta[j];
+            c.tr_W[(size_t)j * P + r] = c.W[j];
+        }
+    }
+    if (j < B * K) {
+        c.s_pi[j] += c.pi[j];
+        if (tr) c.tr_pi[(size_t)j * P + r] = c.pi[j];
+    }
+}
+
+__global__ void k_tick(const ChainDev* __restrict__ cds)
+{
+    if (threadIdx.x == 0) *cds[blockIdx.z].iter += 1;
+}
+
+// ---------------------------------------------------------------------------
+// BIC: per (cell, subtype) log-likelihood row sums (cu:967-1007), one wave each
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bic(const ChainDev* __restrict__ cds, const double* __restrict__ pm,
+                                             double* __restrict__ ll)
+{
+    // pm: posterior means packed as alpha[G] | beta[K*G] | nu[B*G] | delta[N] | phi[B*G] | gamma[2B]
+    const ChainDev& c = cds[blockIdx.z];
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int G = c.G, K = c.K, B = c.B, N = c.N;
+    if (row >= N * K) return;
+    const int i = row / K, k = row % K, b = c.batch[i];
+    const double* al = pm;
+    const double* be = al + G;
+    const double* nu = be + (size_t)K * G;
+    const double* de = nu + (size_t)B * G;
+    const double* phi = de + N;
+    const double* ga = phi + (size_t)B * G;
+    const int32_t* Srow = c.S + (size_t)i * c.Gp;
+    double part = 0.0;
+    for (int g = lane; g < G; g += 64) {
+        int32_t s = Srow[g];
+        int y = s > 0 ? s : 0;
+        part += bq_bic_elem(al[g], be[(size_t)k * G + g], nu[(size_t)b * G + g], de[i], ga[b], ga[B + b],
+                            phi[(size_t)b * G + g], y);
+    }
+    uint64_t a = wave_sum_u64(bq_fix(part));
+    if (lane == 0) ll[(size_t)i * K + k] = bq_unfix(a);
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+#define HIPCHK(x)                                                                     \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) {                                                       \
+            bq_set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+            return BUSSEQ_ERR_DEVICE;                                                 \
+        }                                                                             \
+    } while (0)
+
+struct busseq_sampler {
+    busseq_config cfg{};
+    std::vector<int> n_b;
+    int N = 0, G = 0, Gp = 0, K = 0, B = 0, C = 1, T_max = 0;
+    int prec = 32;
+    hipStream_t stream = nullptr;
+    std::vector<ChainDev> hd;
+    ChainDev* d_cd = nullptr;
+    std::vector<void*> allocs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int iter = 0;
+    int dev = 0;
+
+    template <typename V>
+    V* dalloc(size_t n)
+    {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(V)) != hipSuccess) return nullptr;
+        allocs.push_back(p);
+        if (hipMemset(p, 0, n * sizeof(V)) != hipSuccess) return nullptr;
+        return (V*)p;
+    }
+};
+
+namespace {
+
+template <typename T>
+busseq_status enqueue_sweep(busseq_sampler* s)
+{
+    const dim3 blk(256);
+    const int C = s->C;
+    const int N = s->N, G = s->G, B = s->B, K = s->K;
+    int prep = G;
+    if (B * H > prep) prep = B * H;
+    if (B * K > prep) prep = B * K;
+    const dim3 g_prep((prep + 255) / 256, 1, C);
+    const dim3 g_rows((N + 3) / 4, 1, C);
+    const dim3 g_plan(B, 1, C);
+    const dim3 g_tile((G + 255) / 256, s->T_max, C);
+    const dim3 g_gene((G + 255) / 256, 1, C);
+    const dim3 g_gamma(B + 1, 1, C);
+    int rec = N;
+    if (K * G > rec) rec = K * G;
+    if (B * G > rec) rec = B * G;
+    if (B * K > rec) rec = B * K;
+    if (2 * B > rec) rec = 2 * B;
+    const dim3 g_rec((rec + 255) / 256, 1, C);
+    hipStream_t st = s->stream;
+    const ChainDev* cd = s->d_cd;
+    hipLaunchKernelGGL(k_prep<T>, g_prep, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_w<T>, g_rows, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_plan, g_plan, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_xz_alpha<T>, g_tile, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_alpha_dec<T>, g_gene, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_beta_nu<T>, g_tile, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_bn_dec<T>, g_gene, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_phi_dec<T>, g_gene, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_record, g_rec, blk, 0, st, cd);
+    hipLaunchKernelGGL(k_tick, dim3(1, 1, C), dim3(64), 0, st, cd);
+    HIPCHK(hipGetLastError());
+    return BUSSEQ_OK;
+}
+
+busseq_status build_graph(busseq_sampler* s)
+{
+    if (s->gexec) return BUSSEQ_OK;
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    busseq_status rc = s->prec == 32 ? enqueue_sweep<float>(s) : enqueue_sweep<double>(s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc != BUSSEQ_OK) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) { bq_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return BUSSEQ_ERR_DEVICE; }
+    s->graph = g;
+    HIPCHK(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+    return BUSSEQ_OK;
+}
+
+template <typename T>
+void to_t(std::vector<char>& out, const std::vector<double>& v, double (*f)(double))
+{
+    out.resize(v.size() * sizeof(T));
+    T* p = (T*)out.data();
+    for (size_t j = 0; j < v.size(); ++j) p[j] = (T)(f ? f(v[j]) : v[j]);
+}
+double f_exp(double x) { return bq_exp(x); }
+double f_expm1(double x) { return bq_expm1(x); }
+
+// precision-T copies of a chain's parameters, computed on the host with the
+// same functions the kernels use
+busseq_status upload_tcopies(busseq_sampler* s, int ch, const std::vector<double>& alpha,
+                             const std::vector<double>& beta, const std::vector<double>& nu,
+                             const std::vector<double>& delta, const std::vector<double>& phi)
+{
+    const ChainDev& c = s->hd[ch];
+    std::vector<char> buf;
+    auto up = [&](void* dst) -> hipError_t { return hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice); };
+    if (s->prec == 32) {
+        to_t<float>(buf, alpha, f_exp); HIPCHK(up(c.ea));
+        to_t<float>(buf, beta, f_exp); HIPCHK(up(c.eb));
+        to_t<float>(buf, beta, f_expm1); HIPCHK(up(c.em));
+        to_t<float>(buf, beta, nullptr); HIPCHK(up(c.bt));
+        to_t<float>(buf, nu, f_exp); HIPCHK(up(c.en));
+        to_t<float>(buf, delta, f_exp); HIPCHK(up(c.ed));
+        to_t<float>(buf, phi, nullptr); HIPCHK(up(c.ph));
+    } else {
+        to_t<double>(buf, alpha, f_exp); HIPCHK(up(c.ea));
+        to_t<double>(buf, beta, f_exp); HIPCHK(up(c.eb));
+        to_t<double>(buf, beta, f_expm1); HIPCHK(up(c.em));
+        to_t<double>(buf, beta, nullptr); HIPCHK(up(c.bt));
+        to_t<double>(buf, nu, f_exp); HIPCHK(up(c.en));
+        to_t<double>(buf, delta, f_exp); HIPCHK(up(c.ed));
+        to_t<double>(buf, phi, nullptr); HIPCHK(up(c.ph));
+    }
+    return BUSSEQ_OK;
+}
+
+template <typename V>
+hipError_t h2d(V* dst, const std::vector<V>& v)
+{
+    return hipMemcpy(dst, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice);
+}
+
+busseq_status validate(const busseq_config* cfg)
+{
+    if (!cfg) { bq_set_error("null config"); return BUSSEQ_ERR_ARG; }
+    if (cfg->n_burnin > cfg->n_iter) {
+        bq_set_error("Burn-in iterations must be less than total number of iterations.");
+        return BUSSEQ_ERR_ARG;
+    }
+    if (cfg->B < 2) { bq_set_error("The batch number must be greater than one."); return BUSSEQ_ERR_ARG; }
+    if (cfg->B > BQ_BMAX) { bq_set_error("at most 128 batches are supported"); return BUSSEQ_ERR_ARG; }
+    if (!cfg->n_b) { bq_set_error("batch sizes missing"); return BUSSEQ_ERR_ARG; }
+    for (int b = 0; b < cfg->B; ++b)
+        if (cfg->n_b[b] < cfg->K) {
+            bq_set_error("The sample size in any batch must be greater than the assumed cell type number.");
+            return BUSSEQ_ERR_ARG;
+        }
+    if (cfg->K < 2 || cfg->K > BQ_KMAX) { bq_set_error("K must be between 2 and 16"); return BUSSEQ_ERR_ARG; }
+    if (cfg->G < 1) { bq_set_error("G must be positive"); return BUSSEQ_ERR_ARG; }
+    if (cfg->n_iter < 0 || cfg->n_burnin < 0) { bq_set_error("negative iteration count"); return BUSSEQ_ERR_ARG; }
+    if (cfg->precision != 32 && cfg->precision != 64) { bq_set_error("precision must be 32 or 64"); return BUSSEQ_ERR_ARG; }
+    if (cfg->chains < 1) { bq_set_error("chains must be >= 1"); return BUSSEQ_ERR_ARG; }
+    long long N = 0;
+    for (int b = 0; b < cfg->B; ++b) N += cfg->n_b[b];
+    if (N * (long long)cfg->G > 2147483647LL) { bq_set_error("N x G must stay below 2^31"); return BUSSEQ_ERR_ARG; }
+    return BUSSEQ_OK;
+}
+
+} // namespace
+
+extern "C" busseq_status busseq_create(const busseq_config* cfg, const int32_t* Y, busseq_sampler** out)
+{
+    if (!out) { bq_set_error("null output handle"); return BUSSEQ_ERR_ARG; }
+    *out = nullptr;
+    busseq_status rc = validate(cfg);
+    if (rc != BUSSEQ_OK) return rc;
+    if (!Y) { bq_set_error("null count matrix"); return BUSSEQ_ERR_ARG; }
+    auto* s = new busseq_sampler();
+    s->cfg = *cfg;
+    s->n_b.assign(cfg->n_b, cfg->n_b + cfg->B);
+    s->cfg.n_b = s->n_b.data();
+    s->B = cfg->B; s->G = cfg->G; s->K = cfg->K;
+    s->Gp = (cfg->G + 63) & ~63;
+    s->N = 0;
+    for (int b = 0; b < s->B; ++b) s->N += s->n_b[b];
+    s->C = cfg->chains;
+    s->prec = cfg->precision;
+    s->dev = cfg->device;
+    const int TR = cfg->tile_rows > 0 ? cfg->tile_rows : 64;
+    s->cfg.tile_rows = TR;
+    s->T_max = (s->N + TR - 1) / TR + s->B * s->K;
+    const int N = s->N, G = s->G, Gp = s->Gp, K = s->K, B = s->B;
+    const int P = cfg->n_iter - cfg->n_burnin;
+    auto fail = [&](busseq_status r) { busseq_destroy(s); return r; };
+    if (hipSetDevice(s->dev) != hipSuccess) { bq_set_error("hipSetDevice failed (no GPU?)"); return fail(BUSSEQ_ERR_DEVICE); }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        bq_set_error("hipStreamCreate failed");
+        return fail(BUSSEQ_ERR_DEVICE);
+    }
+    const size_t ts = s->prec == 32 ? 4 : 8;
+    // static data shared by the chains
+    std::vector<int32_t> batch(N), boff(B + 1, 0);
+    for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + s->n_b[b];
+    for (int b = 0; b < B; ++b) for (int i = boff[b]; i < boff[b + 1]; ++i) batch[i] = b;
+    int32_t* d_batch = s->dalloc<int32_t>(N);
+    int32_t* d_boff = s->dalloc<int32_t>(B + 1);
+    int64_t* d_hist = s->dalloc<int64_t>((size_t)B * H);
+    if (!d_batch || !d_boff || !d_hist) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
+    if (h2d(d_batch, batch) != hipSuccess || h2d(d_boff, boff) != hipSuccess) {
+        bq_set_error("upload failed");
+        return fail(BUSSEQ_ERR_DEVICE);
+    }
+    s->hd.resize(s->C);
+    for (int ch = 0; ch < s->C; ++ch) {
+        HostInit hi;
+        uint32_t key = (uint32_t)(cfg->chain0 + ch);
+        if (bq_host_init(B, s->n_b.data(), G, Gp, K, Y, cfg->seed, key, hi) != 0) return fail(BUSSEQ_ERR_DATA);
+        if (ch == 0 && hipMemcpy(d_hist, hi.hist_s.data(), hi.hist_s.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+            bq_set_error("upload failed");
+            return fail(BUSSEQ_ERR_DEVICE);
+        }
+        ChainDev c{};
+        c.N = N; c.G = G; c.Gp = Gp; c.K = K; c.B = B;
+        c.seed = cfg->seed; c.key = key;
+        c.n_burnin = cfg->n_burnin; c.n_unchanged = cfg->n_unchanged;
+        c.n_pres = P; c.traces = cfg->keep_traces ? 1 : 0;
+        c.TR = TR; c.T_max = s->T_max; c.prec = s->prec;
+        c.batch = d_batch; c.boff = d_boff; c.hist_s = d_hist;
+        c.iter = s->dalloc<int>(1);
+        c.S = s->dalloc<int32_t>((size_t)N * Gp);
+        c.W = s->dalloc<int32_t>(N);
+        c.L = s->dalloc<int32_t>((size_t)K * G);
+        c.alpha = s->dalloc<double>(G); c.beta = s->dalloc<double>((size_t)K * G);
+        c.nu = s->dalloc<double>((size_t)B * G); c.delta = s->dalloc<double>(N);
+        c.phi = s->dalloc<double>((size_t)B * G); c.gamma = s->dalloc<double>(2 * (size_t)B);
+        c.pi = s->dalloc<double>((size_t)B * K); c.ptau = s->dalloc<double>(2);
+        c.mu_alpha = s->dalloc<double>(G); c.mu_nu = s->dalloc<double>(B); c.mu_delta = s->dalloc<double>(N);
+        c.ea = s->dalloc<char>(ts * G); c.eb = s->dalloc<char>(ts * K * G); c.em = s->dalloc<char>(ts * K * G);
+        c.bt = s->dalloc<char>(ts * K * G); c.en = s->dalloc<char>(ts * B * G); c.ed = s->dalloc<char>(ts * N);
+        c.ph = s->dalloc<char>(ts * B * G);
+        c.alpha_p = s->dalloc<double>(G); c.beta_p = s->dalloc<double>((size_t)K * G);
+        c.nu_p = s->dalloc<double>((size_t)B * G); c.phi_p = s->dalloc<double>((size_t)B * G);
+        c.gamma_p = s->dalloc<double>(2 * (size_t)B);
+        c.da = s->dalloc<char>(ts * G); c.ca = s->dalloc<char>(ts * G);
+        c.db = s->dalloc<char>(ts * K * G); c.cb = s->dalloc<char>(ts * K * G);
+        c.dn = s->dalloc<char>(ts * B * G); c.cn = s->dalloc<char>(ts * B * G);
+        c.pc = s->dalloc<char>(ts * 6 * B * G);
+        c.cnt = s->dalloc<int32_t>((size_t)B * K);
+        c.perm = s->dalloc<int32_t>(N);
+        c.tiles = s->dalloc<int32_t>((size_t)s->T_max * 4);
+        c.tile_off = s->dalloc<int32_t>((size_t)B * K + 1);
+        c.ntiles = s->dalloc<int32_t>(1);
+        c.part = s->dalloc<uint64_t>((size_t)s->T_max * 3 * Gp);
+        c.hist_d = s->dalloc<uint32_t>((size_t)B * H);
+        c.gstat = s->dalloc<uint64_t>((size_t)B * 5);
+        c.lsum = s->dalloc<int32_t>(G); c.bsq = s->dalloc<uint64_t>(G);
+        c.beta_acc = s->dalloc<int32_t>(G);
+        c.n_rec = s->dalloc<int>(1);
+        c.s_gamma = s->dalloc<double>(2 * (size_t)B); c.s_alpha = s->dalloc<double>(G);
+        c.s_beta = s->dalloc<double>((size_t)K * G); c.s_nu = s->dalloc<double>((size_t)B * G);
+        c.s_delta = s->dalloc<double>(N); c.s_phi = s->dalloc<double>((size_t)B * G);
+        c.s_pi = s->dalloc<double>((size_t)B * K); c.s_ptau = s->dalloc<double>(2);
+        c.c_L = s->dalloc<int32_t>((size_t)K * G); c.c_W = s->dalloc<int32_t>((size_t)N * K);
+        if (c.traces && P > 0) {
+            c.tr_gamma = s->dalloc<double>(2 * (size_t)B * P); c.tr_alpha = s->dalloc<double>((size_t)G * P);
+            c.tr_beta = s->dalloc<double>((size_t)(K - 1) * G * P); c.tr_ptau = s->dalloc<double>(2 * (size_t)P);
+            c.tr_nu = s->dalloc<double>((size_t)(B - 1) * G * P); c.tr_delta = s->dalloc<double>((size_t)N * P);
+            c.tr_phi = s->dalloc<double>((size_t)B * G * P); c.tr_pi = s->dalloc<double>((size_t)B * K * P);
+            c.tr_L = s->dalloc<int32_t>((size_t)(K - 1) * G * P); c.tr_W = s->dalloc<int32_t>((size_t)N * P);
+            if (!c.tr_W || !c.tr_L || !c.tr_delta) { bq_set_error("device allocation of traces failed"); return fail(BUSSEQ_ERR_NOMEM); }
+        }
+        if (!c.part || !c.S || !c.c_W || !c.pc) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
+        s->hd[ch] = c;
+        if (h2d(c.S, hi.S) != hipSuccess || h2d(c.W, hi.W) != hipSuccess || h2d(c.L, hi.L) != hipSuccess ||
+            h2d(c.alpha, hi.alpha) != hipSuccess || h2d(c.beta, hi.beta) != hipSuccess ||
+            h2d(c.nu, hi.nu) != hipSuccess || h2d(c.delta, hi.delta) != hipSuccess ||
+            h2d(c.phi, hi.phi) != hipSuccess || h2d(c.gamma, hi.gamma) != hipSuccess ||
+            h2d(c.pi, hi.pi) != hipSuccess || h2d(c.ptau, hi.ptau) != hipSuccess ||
+            h2d(c.mu_alpha, hi.mu_alpha) != hipSuccess || h2d(c.mu_nu, hi.mu_nu) != hipSuccess ||
+            h2d(c.mu_delta, hi.mu_delta) != hipSuccess) {
+            bq_set_error("upload of the initial state failed");
+            return fail(BUSSEQ_ERR_DEVICE);
+        }
+        rc = upload_tcopies(s, ch, hi.alpha, hi.beta, hi.nu, hi.delta, hi.phi);
+        if (rc != BUSSEQ_OK) return fail(rc);
+    }
+    if (hipMalloc(&s->d_cd, sizeof(ChainDev) * s->C) != hipSuccess) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
+    if (hipMemcpy(s->d_cd, s->hd.data(), sizeof(ChainDev) * s->C, hipMemcpyHostToDevice) != hipSuccess) {
+        bq_set_error("upload failed");
+        return fail(BUSSEQ_ERR_DEVICE);
+    }
+    if (hipDeviceSynchronize() != hipSuccess) { bq_set_error("device sync failed"); return fail(BUSSEQ_ERR_DEVICE); }
+    *out = s;
+    return BUSSEQ_OK;
+}
+
+extern "C" void busseq_destroy(busseq_sampler* s)
+{
+    if (!s) return;
+    hipSetDevice(s->dev);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->gexec) hipGraphExecDestroy(s->gexec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    for (void* p : s->allocs) hipFree(p);
+    if (s->d_cd) hipFree(s->d_cd);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+extern "C" int busseq_iteration(const busseq_sampler* s) { return s ? s->iter : -1; }
+
+extern "C" busseq_status busseq_sweeps(busseq_sampler* s, int n)
+{
+    if (!s || n < 0) { bq_set_error("bad arguments to busseq_sweeps"); return BUSSEQ_ERR_ARG; }
+    HIPCHK(hipSetDevice(s->dev));
+    busseq_status rc = build_graph(s);
+    if (rc != BUSSEQ_OK) return rc;
+    for (int j = 0; j < n; ++j) HIPCHK(hipGraphLaunch(s->gexec, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->iter += n;
+    return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_time_sweeps(busseq_sampler* s, int n, double* ms)
+{
+    if (!s || n < 0 || !ms) { bq_set_error("bad arguments to busseq_time_sweeps"); return BUSSEQ_ERR_ARG; }
+    HIPCHK(hipSetDevice(s->dev));
+    busseq_status rc = build_graph(s);
+    if (rc != BUSSEQ_OK) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipEventRecord(e0, s->stream));
+    for (int j = 0; j < n; ++j) HIPCHK(hipGraphLaunch(s->gexec, s->stream));
+    HIPCHK(hipEventRecord(e1, s->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = f;
+    s->iter += n;
+    return BUSSEQ_OK;
+}
+
+template <typename V>
+static hipError_t d2h(std::vector<V>& v, const V* src, size_t n)
+{
+    v.resize(n);
+    return hipMemcpy(v.data(), src, n * sizeof(V), hipMemcpyDeviceToHost);
+}
+
+extern "C" busseq_status busseq_get_state(busseq_sampler* s, int ch, busseq_state* st)
+{
+    if (!s || !st || ch < 0 || ch >= s->C) { bq_set_error("bad arguments to busseq_get_state"); return BUSSEQ_ERR_ARG; }
+    HIPCHK(hipSetDevice(s->dev));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const ChainDev& c = s->hd[ch];
+    const int N = s->N, G = s->G, Gp = s->Gp, K = s->K, B = s->B;
+    if (st->X || st->Z) {
+        std::vector<int32_t> S;
+        HIPCHK(d2h(S, c.S, (size_t)N * Gp));
+        for (int i = 0; i < N; ++i)
+            for (int g = 0; g < G; ++g) {
+                int32_t v = S[(size_t)i * Gp + g];
+                if (st->X) st->X[(size_t)i * G + g] = v < 0 ? ~v : v;
+                if (st->Z) st->Z[(size_t)i * G + g] = v < 0 ? 1 : 0;
+            }
+    }
+    if (st->W) HIPCHK(hipMemcpy(st->W, c.W, 4 * (size_t)N, hipMemcpyDeviceToHost));
+    if (st->alpha) HIPCHK(hipMemcpy(st->alpha, c.alpha, 8 * (size_t)G, hipMemcpyDeviceToHost));
+    if (st->beta) HIPCHK(hipMemcpy(st->beta, c.beta, 8 * (size_t)K * G, hipMemcpyDeviceToHost));
+    if (st->L) HIPCHK(hipMemcpy(st->L, c.L, 4 * (size_t)K * G, hipMemcpyDeviceToHost));
+    if (st->nu) HIPCHK(hipMemcpy(st->nu, c.nu, 8 * (size_t)B * G, hipMemcpyDeviceToHost));
+    if (st->delta) HIPCHK(hipMemcpy(st->delta, c.delta, 8 * (size_t)N, hipMemcpyDeviceToHost));
+    if (st->phi) HIPCHK(hipMemcpy(st->phi, c.phi, 8 * (size_t)B * G, hipMemcpyDeviceToHost));
+    if (st->gamma) HIPCHK(hipMemcpy(st->gamma, c.gamma, 16 * (size_t)B, hipMemcpyDeviceToHost));
+    if (st->pi) HIPCHK(hipMemcpy(st->pi, c.pi, 8 * (size_t)B * K, hipMemcpyDeviceToHost));
+    if (st->ptau) HIPCHK(hipMemcpy(st->ptau, c.ptau, 16, hipMemcpyDeviceToHost));
+    return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_set_state(busseq_sampler* s, int ch, const busseq_state* st)
+{
+    if (!s || !st || ch < 0 || ch >= s->C) { bq_set_error("bad arguments to busseq_set_state"); return BUSSEQ_ERR_ARG; }
+    HIPCHK(hipSetDevice(s->dev));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const ChainDev& c = s->hd[ch];
+    const int N = s->N, G = s->G, Gp = s->Gp, K = s->K, B = s->B;
+    if (st->X && st->Z) {
+        std::vector<int32_t> S;
+        HIPCHK(d2h(S, c.S, (size_t)N * Gp));
+        for (int i = 0; i < N; ++i)
+            for (int g = 0; g < G; ++g) {
+                int32_t& v = S[(size_t)i * Gp + g];
+                if (v > 0) continue;   // observed count: X = Y, Z = 0 stay fixed
+                int32_t x = st->X[(size_t)i * G + g], z = st->Z[(size_t)i * G + g];
+                v = z ? ~x : 0;
+            }
+        HIPCHK(hipMemcpy(c.S, S.data(), S.size() * 4, hipMemcpyHostToDevice));
+    }
+    auto put = [&](void* dst, const void* src, size_t n) -> hipError_t {
+        return src ? hipMemcpy(dst, src, n, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    HIPCHK(put(c.W, st->W, 4 * (size_t)N));
+    HIPCHK(put(c.alpha, st->alpha, 8 * (size_t)G));
+    HIPCHK(put(c.beta, st->beta, 8 * (size_t)K * G));
+    HIPCHK(put(c.L, st->L, 4 * (size_t)K * G));
+    HIPCHK(put(c.nu, st->nu, 8 * (size_t)B * G));
+    HIPCHK(put(c.delta, st->delta, 8 * (size_t)N));
+    HIPCHK(put(c.phi, st->phi, 8 * (size_t)B * G));
+    HIPCHK(put(c.gamma, st->gamma, 16 * (size_t)B));
+    HIPCHK(put(c.pi, st->pi, 8 * (size_t)B * K));
+    HIPCHK(put(c.ptau, st->ptau, 16));
+    std::vector<double> al, be, nu, de, ph;
+    HIPCHK(d2h(al, c.alpha, G));
+    HIPCHK(d2h(be, c.beta, (size_t)K * G));
+    HIPCHK(d2h(nu, c.nu, (size_t)B * G));
+    HIPCHK(d2h(de, c.delta, N));
+    HIPCHK(d2h(ph, c.phi, (size_t)B * G));
+    return upload_tcopies(s, ch, al, be, nu, de, ph);
+}
+
+extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result* r)
+{
+    if (!s || !r || ch < 0 || ch >= s->C) { bq_set_error("bad arguments to busseq_finish"); return BUSSEQ_ERR_ARG; }
+    HIPCHK(hipSetDevice(s->dev));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    std::memset(r, 0, sizeof(*r));
+    const ChainDev& c = s->hd[ch];
+    const int N = s->N, G = s->G, K = s->K, B = s->B;
+    int nrec = 0;
+    HIPCHK(hipMemcpy(&nrec, c.n_rec, 4, hipMemcpyDeviceToHost));
+    if (nrec <= 0) { bq_set_error("no recorded iterations (n_iter must exceed n_burnin)"); return BUSSEQ_ERR_ARG; }
+    const double n = nrec;
+    auto md = [](size_t k) { return (double*)std::calloc(k ? k : 1, 8); };
+    auto mi = [](size_t k) { return (int32_t*)std::calloc(k ? k : 1, 4); };
+    r->B = B; r->N = N; r->G = G; r->K = K;
+    r->n_pres = c.traces ? (c.n_pres < nrec ? c.n_pres : nrec) : 0;
+    r->n_b = (int*)std::calloc(B, sizeof(int));
+    for (int b = 0; b < B; ++b) r->n_b[b] = s->n_b[b];
+    r->X = mi((size_t)N * G); r->Z = mi((size_t)N * G); r->W = mi(N);
+    r->gamma = md(2 * (size_t)B); r->alpha = md(G); r->L = md((size_t)K * G); r->beta = md((size_t)K * G);
+    r->ptau = md(2); r->nu = md((size_t)B * G); r->delta = md(N); r->phi = md((size_t)B * G); r->pi = md((size_t)B * K);
+    if (!r->X || !r->Z || !r->W) { bq_set_error("host allocation failed"); return BUSSEQ_ERR_NOMEM; }
+    busseq_state st{};
+    st.X = r->X; st.Z = r->Z;
+    busseq_status rc = busseq_get_state(s, ch, &st);
+    if (rc != BUSSEQ_OK) return rc;
+    std::vector<double> v;
+    std::vector<int32_t> vi;
+    HIPCHK(d2h(v, c.s_gamma, 2 * (size_t)B)); for (int j = 0; j < 2 * B; ++j) r->gamma[j] = v[j] / n;
+    HIPCHK(d2h(v, c.s_alpha, G)); for (int j = 0; j < G; ++j) r->alpha[j] = v[j] / n;
+    HIPCHK(d2h(v, c.s_beta, (size_t)K * G)); for (size_t j = 0; j < (size_t)K * G; ++j) r->beta[j] = v[j] / n;
+    HIPCHK(d2h(vi, c.c_L, (size_t)K * G)); for (size_t j = 0; j < (size_t)K * G; ++j) r->L[j] = (double)vi[j] / n;
+    HIPCHK(d2h(v, c.s_nu, (size_t)B * G)); for (size_t j = 0; j < (size_t)B * G; ++j) r->nu[j] = v[j] / n;
+    HIPCHK(d2h(v, c.s_phi, (size_t)B * G)); for (size_t j = 0; j < (size_t)B * G; ++j) r->phi[j] = v[j] / n;
+    HIPCHK(d2h(v, c.s_delta, N)); for (int j = 0; j < N; ++j) r->delta[j] = v[j] / n;
+    HIPCHK(d2h(v, c.s_pi, (size_t)B * K)); for (size_t j = 0; j < (size_t)B * K; ++j) r->pi[j] = v[j] / n;
+    HIPCHK(d2h(v, c.s_ptau, 2)); r->ptau[0] = v[0] / n; r->ptau[1] = v[1] / n;
+    HIPCHK(d2h(vi, c.c_W, (size_t)N * K));
+    for (int i = 0; i < N; ++i) {
+        int best = 0, bv = vi[(size_t)i * K];
+        for (int k = 1; k < K; ++k) if (vi[(size_t)i * K + k] > bv) { bv = vi[(size_t)i * K + k]; best = k; }
+        r->W[i] = best;
+    }
+    // BIC at the posterior means (cu:1815-1833)
+    {
+        std::vector<double> pm;
+        pm.insert(pm.end(), r->alpha, r->alpha + G);
+        pm.insert(pm.end(), r->beta, r->beta + (size_t)K * G);
+        pm.insert(pm.end(), r->nu, r->nu + (size_t)B * G);
+        pm.insert(pm.end(), r->delta, r->delta + N);
+        pm.insert(pm.end(), r->phi, r->phi + (size_t)B * G);
+        pm.insert(pm.end(), r->gamma, r->gamma + 2 * (size_t)B);
+        double* d_pm = nullptr;
+        double* d_ll = nullptr;
+        HIPCHK(hipMalloc(&d_pm, pm.size() * 8));
+        HIPCHK(hipMalloc(&d_ll, (size_t)N * K * 8));
+        HIPCHK(hipMemcpy(d_pm, pm.data(), pm.size() * 8, hipMemcpyHostToDevice));
+        const ChainDev* cd = s->d_cd + ch;
+        hipLaunchKernelGGL(k_bic, dim3(((size_t)N * K + 3) / 4), dim3(256), 0, s->stream, cd, d_pm, d_ll);
+        HIPCHK(hipGetLastError());
+        std::vector<double> ll((size_t)N * K);
+        HIPCHK(hipStreamSynchronize(s->stream));
+        HIPCHK(hipMemcpy(ll.data(), d_ll, ll.size() * 8, hipMemcpyDeviceToHost));
+        hipFree(d_pm);
+        hipFree(d_ll);
+        std::vector<int> batch(N);
+        { int i = 0; for (int b = 0; b < B; ++b) for (int q = 0; q < s->n_b[b]; ++q) batch[i++] = b; }
+        double tot = 0.0;
+        for (int i = 0; i < N; ++i) {
+            int b = batch[i];
+            double mx = -BQ_INF;
+            for (int k = 0; k < K; ++k) if (ll[(size_t)i * K + k] > mx) mx = ll[(size_t)i * K + k];
+            double sm = 0.0;
+            for (int k = 0; k < K; ++k) sm += r->pi[(size_t)b * K + k] * bq_exp(ll[(size_t)i * K + k] - mx);
+            tot += mx + bq_log(sm);
+        }
+        double npar = (double)(B + G) * K + 2.0 * B + (double)G * (B * 2 - 1) + N - B;
+        r->bic = -2.0 * tot + bq_log((double)G * (double)N) * npar;
+    }
+    if (c.traces && r->n_pres > 0) {
+        const size_t P = c.n_pres;
+        auto cpd = [&](double*& dst, const double* src, size_t k) -> hipError_t {
+            dst = md(k * P);
+            return hipMemcpy(dst, src, k * P * 8, hipMemcpyDeviceToHost);
+        };
+        auto cpi = [&](int32_t*& dst, const int32_t* src, size_t k) -> hipError_t {
+            dst = mi(k * P);
+            return hipMemcpy(dst, src, k * P * 4, hipMemcpyDeviceToHost);
+        };
+        HIPCHK(cpd(r->tr_gamma, c.tr_gamma, 2 * (size_t)B));
+        HIPCHK(cpd(r->tr_alpha, c.tr_alpha, G));
+        HIPCHK(cpi(r->tr_L, c.tr_L, (size_t)(K - 1) * G));
+        HIPCHK(cpd(r->tr_beta, c.tr_beta, (size_t)(K - 1) * G));
+        HIPCHK(cpd(r->tr_ptau, c.tr_ptau, 2));
+        HIPCHK(cpd(r->tr_nu, c.tr_nu, (size_t)(B - 1) * G));
+        HIPCHK(cpd(r->tr_delta, c.tr_delta, N));
+        HIPCHK(cpd(r->tr_phi, c.tr_phi, (size_t)B * G));
+        HIPCHK(cpi(r->tr_W, c.tr_W, N));
+        HIPCHK(cpd(r->tr_pi, c.tr_pi, (size_t)B * K));
+    }
+    return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_run(const busseq_config* cfg, const int32_t* Y, busseq_result* out)
+{
+    busseq_sampler* s = nullptr;
+    busseq_status rc = busseq_create(cfg, Y, &s);
+    if (rc != BUSSEQ_OK) return rc;
+    rc = busseq_sweeps(s, cfg->n_iter);
+    if (rc == BUSSEQ_OK) rc = busseq_finish(s, 0, out);
+    busseq_destroy(s);
+    return rc;
+}

@@ -1,0 +1,111 @@
+"""Fit the polynomial coefficients used by include/busseq/numerics.h.
+
+Chebyshev interpolation (near-minimax) of smooth kernels, done in 60-digit
+mpmath arithmetic, converted to the monomial basis and rounded to the target
+precision.  Prints C initialisers.  Run: python busseq_gpu_amd/csrc/gen_coeffs.py
+"""
+import mpmath as mp
+
+mp.mp.dps = 60
+
+
+def cheb_fit(fn, a, b, deg):
+    a, b = mp.mpf(a), mp.mpf(b)
+    n = deg + 1
+    nodes = [mp.cos(mp.pi * (k + mp.mpf(1) / 2) / n) for k in range(n)]
+    xs = [(b - a) / 2 * t + (a + b) / 2 for t in nodes]
+    fx = [fn(x) for x in xs]
+    # Chebyshev coefficients on [-1,1]
+    c = []
+    for j in range(n):
+        s = mp.fsum(fx[k] * mp.cos(j * mp.pi * (k + mp.mpf(1) / 2) / n) for k in range(n))
+        c.append(2 * s / n)
+    c[0] /= 2
+    # convert sum c_j T_j(t) with t=(2x-a-b)/(b-a) to monomials in x
+    # build T_j as polynomials in t, then substitute
+    T = [[mp.mpf(1)], [mp.mpf(0), mp.mpf(1)]]
+    for j in range(2, n):
+        p = [mp.mpf(0)] + [2 * v for v in T[j - 1]]
+        for i, v in enumerate(T[j - 2]):
+            p[i] -= v
+        T.append(p)
+    poly_t = [mp.mpf(0)] * n
+    for j in range(n):
+        for i, v in enumerate(T[j]):
+            poly_t[i] += c[j] * v
+    # t = alpha*x + beta
+    alpha = 2 / (b - a)
+    beta = -(a + b) / (b - a)
+    poly_x = [mp.mpf(0)] * n
+    # expand (alpha x + beta)^i
+    for i, ci in enumerate(poly_t):
+        for m in range(i + 1):
+            poly_x[m] += ci * mp.binomial(i, m) * alpha ** m * beta ** (i - m)
+    return poly_x
+
+
+def horner(coefs, x):
+    r = mp.mpf(0)
+    for cf in reversed(coefs):
+        r = r * x + cf
+    return r
+
+
+def report(name, fn, a, b, deg, cast, rel_fn):
+    co = cheb_fit(fn, a, b, deg)
+    co_r = [mp.mpf(cast(float(v))) for v in co]
+    worst = mp.mpf(0)
+    for k in range(4001):
+        x = mp.mpf(a) + (mp.mpf(b) - mp.mpf(a)) * k / 4000
+        if x == 0:
+            continue
+        err = rel_fn(x, horner(co_r, x))
+        worst = max(worst, abs(err))
+    return co_r, worst
+
+
+def log1p_q(x):
+    # Q(f) = (log1p(f) - f) / f^2
+    if x == 0:
+        return mp.mpf(-0.5)
+    return (mp.log1p(x) - x) / x ** 2
+
+
+def exp_e(x):
+    # E(r) = (exp(r) - 1 - r) / r^2
+    if x == 0:
+        return mp.mpf(0.5)
+    return (mp.expm1(x) - x) / x ** 2
+
+
+def main():
+    import numpy as np
+    f32 = lambda v: float(np.float32(v))
+    f64 = lambda v: float(v)
+    lo, hi = mp.sqrt(0.5) - 1, mp.sqrt(2) - 1
+    lo = lo - mp.mpf("1e-3")
+    hi = hi + mp.mpf("1e-3")
+
+    def rel_log(x, q):
+        return (x + x * x * q - mp.log1p(x)) / mp.log1p(x)
+
+    def rel_exp(x, e):
+        return (1 + x + x * x * e - mp.exp(x)) / mp.exp(x)
+
+    out = []
+    for name, cast, deg in (("LOG1P_Q_F32", f32, 8), ("LOG1P_Q_F64", f64, 19)):
+        co, w = report(name, log1p_q, lo, hi, deg, cast, rel_log)
+        out.append((name, co, w))
+    r = mp.log(2) / 2 + mp.mpf("1e-3")
+    for name, cast, deg in (("EXP_E_F64", f64, 11),):
+        co, w = report(name, exp_e, -r, r, deg, cast, rel_exp)
+        out.append((name, co, w))
+    for name, co, w in out:
+        print(f"/* {name}: degree {len(co)-1}, max rel err of the rounded polynomial {mp.nstr(w, 3)} */")
+        print(f"#define {name} \\")
+        print("    " + ", \\\n    ".join(mp.nstr(v, 21, min_fixed=0, max_fixed=0) for v in co))
+        print()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,433 @@
+// host.cpp -- host side of libbusseq: initial state (BUSseq_gpu.cu:1163-1244),
+// input readers (cu:1094-1098, 1145-1159), output writers (cu:1610-2018),
+// the synthetic-data generator and error reporting.  Compiled with g++
+// (-ffp-contract=off) so that the initial state uses the same numerics as the
+// device code.
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../../include/busseq.h"
+#include "../../include/busseq/model.h"
+#include "engine.h"
+
+static thread_local std::string g_err;
+
+void bq_set_error(const std::string& m) { g_err = m; }
+
+extern "C" const char* busseq_last_error(void) { return g_err.c_str(); }
+
+extern "C" void busseq_config_default(busseq_config* c)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->B = 4; c->n_b = nullptr; c->G = 3000; c->K = 5;
+    c->seed = 0; c->n_iter = 4000; c->n_burnin = 2000; c->n_unchanged = 500;
+    c->keep_traces = 0; c->precision = 32; c->chains = 1; c->chain0 = 0;
+    c->tile_rows = 64; c->device = 0;
+}
+
+// ---------------------------------------------------------------------------
+// initial state, BUSseq_gpu.cu:1163-1244
+// ---------------------------------------------------------------------------
+int bq_host_init(int B, const int* n_b, int G, int Gp, int K, const int32_t* Y,
+                 uint32_t seed, uint32_t chain, HostInit& o)
+{
+    const int H = BQ_HIST;
+    std::vector<int> boff(B + 1, 0), batch;
+    for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + n_b[b];
+    const int N = boff[B];
+    batch.resize(N);
+    for (int b = 0; b < B; ++b)
+        for (int i = boff[b]; i < boff[b + 1]; ++i) batch[i] = b;
+
+    o.S.assign((size_t)N * Gp, 0);
+    o.hist_s.assign((size_t)B * H, 0);
+    std::vector<int64_t> lib(N, 0);
+    for (int i = 0; i < N; ++i) {
+        int64_t s = 0;
+        for (int g = 0; g < G; ++g) {
+            int32_t y = Y[(size_t)i * G + g];
+            int32_t v = y > 0 ? y : 0;
+            o.S[(size_t)i * Gp + g] = v;
+            s += v;
+            if (v > 0 && v < H) o.hist_s[(size_t)batch[i] * H + v] += 1;
+        }
+        lib[i] = s;
+        if (s <= 0) {
+            bq_set_error("cell " + std::to_string(i) + " has no counts (zero library size)");
+            return -1;
+        }
+    }
+    // pi0_bk = (k+1)/(K(K+1)/2)  (cu:392-397)
+    o.pi.assign((size_t)B * K, 0.0);
+    for (int b = 0; b < B; ++b)
+        for (int k = 0; k < K; ++k) o.pi[(size_t)b * K + k] = (k + 1) / (K * (K + 1) / 2.0);
+    // W0 ~ Cat(pi0_b); delta0_i = log s_i - log s_first(b)  (cu:419-443)
+    o.W.assign(N, 0);
+    o.delta.assign(N, 0.0);
+    for (int i = 0; i < N; ++i) {
+        int b = batch[i];
+        bq_rng r;
+        bq_rng_init(&r, seed, chain, (uint32_t)i, 0u, BQ_SWEEP_INIT, BQ_PH_INIT_W);
+        double u = bq_uniform(&r);
+        int k = 0;
+        double tmp = o.pi[(size_t)b * K];
+        while (u > tmp && k < K - 1) { k++; tmp += o.pi[(size_t)b * K + k]; }
+        o.W[i] = k;
+        o.delta[i] = bq_log((double)lib[i]) - bq_log((double)lib[boff[b]]);
+    }
+    // raw means r_bkg (cu:369-382, 1210-1221) as exact sums of log(1 + Y e^-delta)
+    std::vector<uint64_t> rs((size_t)B * K * G, 0);
+    std::vector<int> nbk((size_t)B * K, 0);
+    for (int i = 0; i < N; ++i) {
+        int b = batch[i], k = o.W[i];
+        nbk[(size_t)b * K + k] += 1;
+        double ie = bq_exp(-o.delta[i]);
+        uint64_t* row = &rs[((size_t)b * K + k) * G];
+        for (int g = 0; g < G; ++g) {
+            int32_t y = o.S[(size_t)i * Gp + g];
+            row[g] += bq_fix(bq_log1p((double)y * ie));
+        }
+    }
+    std::vector<double> rm((size_t)B * K * G, 0.0);
+    for (int b = 0; b < B; ++b)
+        for (int k = 0; k < K; ++k)
+            for (int g = 0; g < G; ++g) {
+                size_t j = ((size_t)b * K + k) * G + g;
+                int n = nbk[(size_t)b * K + k];
+                rm[j] = n == 0 ? 0.0 : bq_unfix(rs[j]) / n;
+            }
+    // mu_nu_b (cu:470-482, 1232-1241)
+    std::vector<double> bm(B, 0.0);
+    for (int b = 0; b < B; ++b) {
+        unsigned __int128 tot = 0;
+        for (int k = 0; k < K; ++k)
+            for (int g = 0; g < G; ++g)
+                tot += (unsigned __int128)(__int128)(int64_t)rs[((size_t)b * K + k) * G + g];
+        bm[b] = (double)(__int128)tot * BQ_FIX_INV / ((double)n_b[b] * G);
+    }
+    o.mu_nu.assign(B, 0.0);
+    for (int b = 1; b < B; ++b) o.mu_nu[b] = bm[b] - bm[0];
+    // gamma0 = (0, -0.3) (cu:384-390)
+    o.gamma.assign(2 * (size_t)B, 0.0);
+    for (int b = 0; b < B; ++b) { o.gamma[b] = BQ_GAMMA0_INIT; o.gamma[B + b] = BQ_GAMMA1_INIT; }
+    // phi0 = 5, nu0 (cu:399-417)
+    o.phi.assign((size_t)B * G, BQ_PHI_INIT);
+    o.nu.assign((size_t)B * G, 0.0);
+    for (int b = 1; b < B; ++b)
+        for (int g = 0; g < G; ++g) {
+            double t1 = rm[((size_t)b * K) * G + g], t2 = rm[g];
+            for (int k = 1; k < K; ++k) {
+                t1 += rm[((size_t)b * K + k) * G + g];
+                t2 += rm[(size_t)k * G + g];
+            }
+            o.nu[(size_t)b * G + g] = (t1 - t2) / K;
+        }
+    // p0 ~ U(0, 0.5) from the Philox stream, tau0 = 0.005 (cu:1172-1175)
+    o.ptau.assign(2, 0.0);
+    {
+        bq_rng r;
+        bq_rng_init(&r, seed, chain, 0u, 0u, BQ_SWEEP_INIT, BQ_PH_INIT_P);
+        o.ptau[0] = bq_uniform(&r) * BQ_P_INIT_BOUND;
+        o.ptau[1] = BQ_TAU0_INIT;
+    }
+    // alpha0, beta0, L0 (cu:445-462, 1186, 1227-1230)
+    double p = o.ptau[0];
+    double lrb = bq_log(p / (1.0 - p)) + (bq_log(BQ_TAU0_INIT) - bq_log(BQ_TAU1)) / 2.0;
+    o.alpha.assign(G, 0.0);
+    o.mu_alpha.assign(G, 0.0);
+    o.beta.assign((size_t)K * G, 0.0);
+    o.L.assign((size_t)K * G, 0);
+    for (int g = 0; g < G; ++g) { o.alpha[g] = rm[g]; o.mu_alpha[g] = rm[g]; }
+    for (int k = 1; k < K; ++k)
+        for (int g = 0; g < G; ++g) {
+            size_t j = (size_t)k * G + g;
+            o.beta[j] = rm[j] - rm[g];
+            double lr = lrb + (o.beta[j] * o.beta[j]) * (1.0 / BQ_TAU0_INIT - 1.0 / BQ_TAU1) / 2.0;
+            bq_rng r;
+            bq_rng_init(&r, seed, chain, (uint32_t)g, (uint32_t)k, BQ_SWEEP_INIT, BQ_PH_INIT_L);
+            o.L[j] = bq_bern_logodds(bq_uniform(&r), lr);
+        }
+    o.mu_delta = o.delta;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// input (cu:1094-1098, 1145-1159), strict: malformed or short input is an error
+// ---------------------------------------------------------------------------
+static bool read_file(const char* path, std::vector<char>& buf)
+{
+    FILE* f = std::fopen(path, "rb");
+    if (!f) { bq_set_error(std::string("cannot open ") + path + ": " + std::strerror(errno)); return false; }
+    std::fseek(f, 0, SEEK_END);
+    long n = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (n < 0) { std::fclose(f); bq_set_error(std::string("cannot size ") + path); return false; }
+    buf.resize((size_t)n + 1);
+    size_t got = n ? std::fread(buf.data(), 1, (size_t)n, f) : 0;
+    std::fclose(f);
+    if (got != (size_t)n) { bq_set_error(std::string("short read of ") + path); return false; }
+    buf[(size_t)n] = 0;
+    return true;
+}
+
+// parse up to `want` whitespace-separated non-negative ints; returns count parsed, -1 on a bad token
+static long parse_ints(const char* p, const char* end, long want, int32_t* out, std::string& bad)
+{
+    long n = 0;
+    while (n < want) {
+        while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r' || *p == '\f' || *p == '\v')) ++p;
+        if (p >= end) break;
+        const char* tok = p;
+        bool neg = false;
+        if (*p == '+' || *p == '-') { neg = *p == '-'; ++p; }
+        int64_t v = 0;
+        const char* d0 = p;
+        while (p < end && *p >= '0' && *p <= '9') { v = v * 10 + (*p - '0'); if (v > 2147483647LL) break; ++p; }
+        bool ws_end = p >= end || *p == ' ' || *p == '\t' || *p == '\n' || *p == '\r' || *p == '\f' || *p == '\v';
+        if (p == d0 || !ws_end || v > 2147483647LL) {
+            const char* q = tok;
+            while (q < end && q - tok < 24 && !(*q == ' ' || *q == '\t' || *q == '\n' || *q == '\r')) ++q;
+            bad.assign(tok, q);
+            return -1;
+        }
+        out[n++] = (int32_t)(neg ? -v : v);
+    }
+    return n;
+}
+
+extern "C" busseq_status busseq_read_dims(const char* path, int B, int* n_b)
+{
+    std::vector<char> buf;
+    if (!read_file(path, buf)) return BUSSEQ_ERR_IO;
+    std::string bad;
+    std::vector<int32_t> v(B > 0 ? B : 1);
+    long n = parse_ints(buf.data(), buf.data() + buf.size() - 1, B, v.data(), bad);
+    if (n < 0) { bq_set_error(std::string(path) + ": not an integer: '" + bad + "'"); return BUSSEQ_ERR_FORMAT; }
+    if (n < B) {
+        bq_set_error(std::string(path) + ": expected " + std::to_string(B) + " batch sizes, found " + std::to_string(n));
+        return BUSSEQ_ERR_FORMAT;
+    }
+    for (int b = 0; b < B; ++b) n_b[b] = v[b];
+    return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_read_counts(const char* path, int N, int G, int32_t* Y)
+{
+    std::vector<char> buf;
+    if (!read_file(path, buf)) return BUSSEQ_ERR_IO;
+    long want = (long)N * G;
+    std::string bad;
+    long n = parse_ints(buf.data(), buf.data() + buf.size() - 1, want, Y, bad);
+    if (n < 0) { bq_set_error(std::string(path) + ": not an integer count: '" + bad + "'"); return BUSSEQ_ERR_FORMAT; }
+    if (n < want) {
+        bq_set_error(std::string(path) + ": expected " + std::to_string(want) + " counts (N x G), found " + std::to_string(n));
+        return BUSSEQ_ERR_FORMAT;
+    }
+    for (long j = 0; j < want; ++j)
+        if (Y[j] < 0) {
+            bq_set_error(std::string(path) + ": negative count at cell " + std::to_string(j / G) + ", gene " + std::to_string(j % G));
+            return BUSSEQ_ERR_FORMAT;
+        }
+    return BUSSEQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// synthetic data from the BUSseq generative model
+// ---------------------------------------------------------------------------
+extern "C" busseq_status busseq_simulate(int B, const int* n_b, int G, int K, uint32_t ds,
+                                         double g0_lo, double g0_hi, int32_t* Y, int32_t* W_true,
+                                         int32_t* X_true)
+{
+    if (B < 1 || G < 1 || K < 1 || !n_b) { bq_set_error("simulate: bad dimensions"); return BUSSEQ_ERR_ARG; }
+    const uint32_t key = 0x5EEDu;
+    std::vector<int> boff(B + 1, 0);
+    for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + n_b[b];
+    const int N = boff[B];
+    std::vector<double> alpha(G), beta((size_t)K * G, 0.0), nu((size_t)B * G, 0.0), phi((size_t)B * G);
+    std::vector<double> g0(B), g1(B), pi((size_t)B * K);
+    for (int g = 0; g < G; ++g) {
+        bq_rng r; bq_rng_init(&r, ds, key, (uint32_t)g, 0u, 0u, BQ_PH_SIM_PARAM);
+        alpha[g] = 1.5 + 0.5 * bq_normal(&r);
+    }
+    int m = (int)(0.1 * G + 0.5);
+    std::vector<int> idx(G);
+    for (int k = 1; k < K; ++k) {
+        bq_rng r; bq_rng_init(&r, ds, key, (uint32_t)k, 1u, 0u, BQ_PH_SIM_PARAM);
+        for (int g = 0; g < G; ++g) idx[g] = g;
+        for (int j = 0; j < m; ++j) {
+            int s = j + (int)(bq_uniform(&r) * (G - j));
+            if (s >= G) s = G - 1;
+            int t = idx[j]; idx[j] = idx[s]; idx[s] = t;
+            double sign = bq_uniform(&r) < 0.5 ? -1.0 : 1.0;
+            beta[(size_t)k * G + idx[j]] = 2.0 * sign + 0.5 * bq_normal(&r);
+        }
+    }
+    for (int b = 0; b < B; ++b) {
+        for (int g = 0; g < G; ++g) {
+            bq_rng r; bq_rng_init(&r, ds, key, (uint32_t)g, 16u + (uint32_t)b, 0u, BQ_PH_SIM_PARAM);
+            nu[(size_t)b * G + g] = b == 0 ? 0.0 : 0.5 * bq_normal(&r);
+            phi[(size_t)b * G + g] = bq_rgamma(&r, 5.0, 2.0);
+        }
+        bq_rng r; bq_rng_init(&r, ds, key, (uint32_t)b, 400u, 0u, BQ_PH_SIM_PARAM);
+        g0[b] = g0_lo + (g0_hi - g0_lo) * bq_uniform(&r);
+        g1[b] = -0.3 + 0.2 * bq_uniform(&r);
+        double tot = 0.0;
+        for (int k = 0; k < K; ++k) { pi[(size_t)b * K + k] = bq_rgamma(&r, 1.0, 1.0); tot += pi[(size_t)b * K + k]; }
+        for (int k = 0; k < K; ++k) pi[(size_t)b * K + k] /= tot;
+    }
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < N; ++i) {
+        int b = 0;
+        while (i >= boff[b + 1]) ++b;
+        bq_rng rc; bq_rng_init(&rc, ds, key, (uint32_t)i, 0u, 0u, BQ_PH_SIM_CELL);
+        double delta = 0.2 * bq_normal(&rc);
+        if (i == boff[b]) delta = 0.0;
+        double u = bq_uniform(&rc), cum = 0.0;
+        int w = K - 1;
+        for (int k = 0; k < K; ++k) { cum += pi[(size_t)b * K + k]; if (u <= cum) { w = k; break; } }
+        if (W_true) W_true[i] = w;
+        for (int g = 0; g < G; ++g) {
+            bq_rng r; bq_rng_init(&r, ds, key, (uint32_t)i, (uint32_t)g, 0u, BQ_PH_SIM_COUNT);
+            double mu = bq_exp(alpha[g] + beta[(size_t)w * G + g] + nu[(size_t)b * G + g] + delta);
+            int x = bq_rnbinom(&r, mu, phi[(size_t)b * G + g]);
+            double pz = 1.0 / (1.0 + bq_exp(-(g0[b] + g1[b] * x)));
+            int z = bq_uniform(&r) < pz;
+            if (Y) Y[(size_t)i * G + g] = z ? 0 : x;
+            if (X_true) X_true[(size_t)i * G + g] = x;
+        }
+    }
+    return BUSSEQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// output writers, byte-compatible with BUSseq_gpu.cu:1610-2018
+// ---------------------------------------------------------------------------
+namespace {
+struct Out {
+    FILE* f = nullptr;
+    std::vector<char> buf;
+    bool ok = true;
+    explicit Out(const std::string& path)
+    {
+        f = std::fopen(path.c_str(), "w");
+        if (!f) { ok = false; bq_set_error("cannot write " + path + ": " + std::strerror(errno)); }
+        buf.reserve(1 << 20);
+    }
+    void flush()
+    {
+        if (f && !buf.empty()) { if (std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) ok = false; buf.clear(); }
+    }
+    void put(const char* s, size_t n) { buf.insert(buf.end(), s, s + n); if (buf.size() > (1u << 20)) flush(); }
+    void i32(int32_t v, char sep)
+    {
+        char t[16]; int n = 0;
+        uint32_t u = v < 0 ? (uint32_t)(-(int64_t)v) : (uint32_t)v;
+        char d[12]; int m = 0;
+        do { d[m++] = (char)('0' + u % 10); u /= 10; } while (u);
+        if (v < 0) t[n++] = '-';
+        while (m) t[n++] = d[--m];
+        t[n++] = sep;
+        put(t, (size_t)n);
+    }
+    void f64(double v, char sep)
+    {
+        char t[400];
+        int n = std::snprintf(t, sizeof(t), "%lf", v);
+        t[n++] = sep;
+        put(t, (size_t)n);
+    }
+    bool close()
+    {
+        flush();
+        if (f && std::fclose(f) != 0) ok = false;
+        f = nullptr;
+        return ok;
+    }
+    ~Out() { if (f) std::fclose(f); }
+};
+
+// rows x cols matrix, tab-separated, newline per row
+template <typename V>
+bool write_matrix(const std::string& path, const V* a, long rows, long cols)
+{
+    Out o(path);
+    if (!o.ok) return false;
+    for (long r = 0; r < rows; ++r)
+        for (long c = 0; c < cols; ++c) {
+            char sep = c == cols - 1 ? '\n' : '\t';
+            if constexpr (sizeof(V) == 4) o.i32((int32_t)a[r * cols + c], sep); else o.f64((double)a[r * cols + c], sep);
+        }
+    return o.close();
+}
+
+// matrix with a literal all-zero first row (cu:1705-1707, 1740-1742)
+bool write_zero_row_matrix(const std::string& path, const double* rest, long rows_rest, long cols)
+{
+    Out o(path);
+    if (!o.ok) return false;
+    for (long c = 0; c < cols; ++c) o.put(c == cols - 1 ? "0.000000\n" : "0.000000\t", 9);
+    for (long r = 0; r < rows_rest; ++r)
+        for (long c = 0; c < cols; ++c) o.f64(rest[r * cols + c], c == cols - 1 ? '\n' : '\t');
+    return o.close();
+}
+} // namespace
+
+extern "C" busseq_status busseq_write_outputs(const busseq_result* r, const char* prefix, int print_preserved)
+{
+    if (!r || !prefix) { bq_set_error("write_outputs: null argument"); return BUSSEQ_ERR_ARG; }
+    const std::string p(prefix);
+    const long B = r->B, N = r->N, G = r->G, K = r->K, P = r->n_pres;
+    bool ok = true;
+    ok = ok && write_matrix(p + "_post_Z.txt", r->Z, N, G);
+    ok = ok && write_matrix(p + "_post_X.txt", r->X, N, G);
+    ok = ok && write_matrix(p + "_post_gamma.txt", r->gamma, 2, B);
+    ok = ok && write_matrix(p + "_post_alpha.txt", r->alpha, G, 1);
+    ok = ok && write_matrix(p + "_post_L.txt", r->L + G, K - 1, G);
+    ok = ok && write_zero_row_matrix(p + "_post_beta.txt", r->beta + G, K - 1, G);
+    if (ok) {
+        Out o(p + "_post_p.txt");
+        ok = o.ok;
+        if (ok) { o.f64(r->ptau[0], '\t'); o.f64(r->ptau[1], '\t'); ok = o.close(); }
+    }
+    ok = ok && write_zero_row_matrix(p + "_post_nu.txt", r->nu + G, B - 1, G);
+    ok = ok && write_matrix(p + "_post_delta.txt", r->delta, N, 1);
+    ok = ok && write_matrix(p + "_post_phi.txt", r->phi, B, G);
+    ok = ok && write_matrix(p + "_post_W.txt", r->W, N, 1);
+    ok = ok && write_matrix(p + "_post_pi.txt", r->pi, B, K);
+    if (ok) {
+        Out o(p + "_BIC.txt");
+        ok = o.ok;
+        if (ok) { o.f64(r->bic, '\n'); ok = o.close(); }
+    }
+    if (ok && print_preserved) {
+        if (!r->tr_alpha) { bq_set_error("write_outputs: traces were not kept (keep_traces = 0)"); return BUSSEQ_ERR_ARG; }
+        ok = ok && write_matrix(p + "_preserved_gamma.txt", r->tr_gamma, 2 * B, P);
+        ok = ok && write_matrix(p + "_preserved_alpha.txt", r->tr_alpha, G, P);
+        ok = ok && write_matrix(p + "_preserved_L.txt", r->tr_L, (K - 1) * G, P);
+        ok = ok && write_matrix(p + "_preserved_beta.txt", r->tr_beta, G * (K - 1), P);
+        ok = ok && write_matrix(p + "_preserved_p.txt", r->tr_ptau, 2, P);
+        ok = ok && write_matrix(p + "_preserved_nu.txt", r->tr_nu, (B - 1) * G, P);
+        ok = ok && write_matrix(p + "_preserved_delta.txt", r->tr_delta, N, P);
+        ok = ok && write_matrix(p + "_preserved_phi.txt", r->tr_phi, B * G, P);
+        ok = ok && write_matrix(p + "_preserved_W.txt", r->tr_W, N, P);
+        ok = ok && write_matrix(p + "_preserved_pi.txt", r->tr_pi, B * K, P);
+    }
+    return ok ? BUSSEQ_OK : BUSSEQ_ERR_IO;
+}
+
+extern "C" void busseq_result_free(busseq_result* r)
+{
+    if (!r) return;
+    void* ps[] = { r->n_b, r->X, r->Z, r->W, r->gamma, r->alpha, r->L, r->beta, r->ptau, r->nu, r->delta,
+                   r->phi, r->pi, r->tr_gamma, r->tr_alpha, r->tr_L, r->tr_beta, r->tr_ptau, r->tr_nu,
+                   r->tr_delta, r->tr_phi, r->tr_W, r->tr_pi };
+    for (void* q : ps) std::free(q);
+    std::memset(r, 0, sizeof(*r));
+}

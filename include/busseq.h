@@ -1,0 +1,122 @@
+/*
+ * busseq.h -- C ABI of the MI355X-native BUSseq Gibbs sampler (libbusseq.so).
+ *
+ * The reference (BUSseq_gpu.cu) has no library API: its boundary is the
+ * executable `BUSseq_gpu -B -N -G -K -s -c -i -b -u -p -o` and its text
+ * outputs (BUSseq_gpu.cu:1026-2021, README.md:17-49).  This header is the
+ * library form of that boundary: plain pointers and sizes, status codes
+ * instead of exit(), one sampler handle per host thread.  The bundled
+ * executable BUSseq_gpu is a thin CLI over these calls.
+ *
+ * Layout conventions follow the reference (BUSseq_gpu.cu:1148, 1177-1189):
+ *   counts Y, X, Z   N x G int32, row-major [i*G + g], cells grouped by batch
+ *   beta, L          K x G  [k*G + g]   (row k = 0 is the reference subtype)
+ *   nu, phi          B x G  [b*G + g]   (nu row b = 0 is zero)
+ *   pi               B x K  [b*K + k]
+ *   gamma            2 x B  [0*B + b] = gamma0, [1*B + b] = gamma1
+ *   ptau             2      p, tau0
+ */
+#ifndef BUSSEQ_H
+#define BUSSEQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BUSSEQ_OK = 0,
+    BUSSEQ_ERR_ARG = 1,      /* invalid configuration (cu:1120-1134 checks and more) */
+    BUSSEQ_ERR_IO = 2,       /* file could not be opened or written */
+    BUSSEQ_ERR_FORMAT = 3,   /* count or dimension file malformed or short */
+    BUSSEQ_ERR_DATA = 4,     /* data the model cannot use (e.g. a cell with zero counts) */
+    BUSSEQ_ERR_DEVICE = 5,   /* HIP runtime error */
+    BUSSEQ_ERR_NOMEM = 6     /* host or device allocation failed */
+} busseq_status;
+
+typedef struct busseq_config {
+    int B;                /* batches (-B) */
+    const int* n_b;       /* cells per batch, B entries (-N file) */
+    int G;                /* genes (-G) */
+    int K;                /* subtypes (-K), 2 <= K <= 16 */
+    uint32_t seed;        /* -s */
+    int n_iter;           /* -i */
+    int n_burnin;         /* -b */
+    int n_unchanged;      /* -u: p and tau0 are updated from this iteration on */
+    int keep_traces;      /* -p: keep every recorded iteration */
+    int precision;        /* 32: single-precision per-element terms (default); 64: double */
+    int chains;           /* independent chains on the same data in one launch (default 1) */
+    int chain0;           /* Philox key word of the first chain (default 0) */
+    int tile_rows;        /* cells per reduction tile (default 64) */
+    int device;           /* HIP device ordinal (default 0) */
+} busseq_config;
+
+/* fills the reference defaults of BUSseq_gpu.cu:1028-1038 (n_b left NULL) */
+void busseq_config_default(busseq_config* cfg);
+
+/* one chain's state; caller-allocated arrays of the documented sizes, NULL = skip */
+typedef struct busseq_state {
+    int32_t* X; int32_t* Z; int32_t* W;
+    double* alpha; double* beta; int32_t* L; double* nu; double* delta; double* phi;
+    double* gamma; double* pi; double* ptau;
+} busseq_state;
+
+/* posterior summaries; library-allocated, released by busseq_result_free */
+typedef struct busseq_result {
+    int B, N, G, K, n_pres;
+    int* n_b;
+    int32_t* X; int32_t* Z;   /* last iteration (cu:1614-1645) */
+    int32_t* W;               /* posterior mode, lowest k on ties (cu:944-959) */
+    double* gamma; double* alpha; double* L; double* beta; double* ptau;
+    double* nu; double* delta; double* phi; double* pi;   /* posterior means; L = P(L=1) */
+    double bic;               /* cu:967-1024, 1815-1833 */
+    /* every recorded iteration when keep_traces: [param * n_pres + iter] (cu:892-936) */
+    double* tr_gamma; double* tr_alpha; int32_t* tr_L; double* tr_beta; double* tr_ptau;
+    double* tr_nu; double* tr_delta; double* tr_phi; int32_t* tr_W; double* tr_pi;
+} busseq_result;
+
+typedef struct busseq_sampler busseq_sampler;
+
+/* Whole run: init + n_iter sweeps + posterior + BIC.  Replaces main() of
+ * BUSseq_gpu.cu:1163-1843 (everything after argument parsing and file input). */
+busseq_status busseq_run(const busseq_config* cfg, const int32_t* Y, busseq_result* out);
+
+/* Text outputs <prefix>_post_*.txt, _BIC.txt and, if print_preserved,
+ * _preserved_*.txt in the reference's formats (BUSseq_gpu.cu:1610-2018). */
+busseq_status busseq_write_outputs(const busseq_result* r, const char* prefix, int print_preserved);
+void busseq_result_free(busseq_result* r);
+
+/* Stepping interface (same semantics as busseq_run, split for tests and timing).
+ * create = allocation + initialisation (BUSseq_gpu.cu:1163-1244). */
+busseq_status busseq_create(const busseq_config* cfg, const int32_t* Y, busseq_sampler** out);
+/* n Gibbs sweeps (the loops of BUSseq_gpu.cu:1267-1555), replayed from one hipGraph */
+busseq_status busseq_sweeps(busseq_sampler* s, int n);
+/* n sweeps timed with HIP events on the sampler's stream; *ms = elapsed milliseconds */
+busseq_status busseq_time_sweeps(busseq_sampler* s, int n, double* ms);
+int busseq_iteration(const busseq_sampler* s);
+busseq_status busseq_get_state(busseq_sampler* s, int chain, busseq_state* st);
+busseq_status busseq_set_state(busseq_sampler* s, int chain, const busseq_state* st);
+/* posterior + BIC of one chain after the run (BUSseq_gpu.cu:1557-1608, 1815-1833) */
+busseq_status busseq_finish(busseq_sampler* s, int chain, busseq_result* out);
+void busseq_destroy(busseq_sampler* s);
+
+/* Input files.  -N file: B whitespace-separated ints (cu:1094-1098).
+ * -c file: N*G whitespace-separated ints, cells as rows (cu:1145-1159). */
+busseq_status busseq_read_dims(const char* path, int B, int* n_b);
+busseq_status busseq_read_counts(const char* path, int N, int G, int32_t* Y);
+
+/* Synthetic counts from the BUSseq generative model (see DESIGN of the
+ * benchmark configurations).  gamma0 ~ U(g0_lo, g0_hi), gamma1 ~ U(-0.3,-0.1).
+ * Outputs that are NULL are skipped: Y (N*G), true W (N), X (N*G). */
+busseq_status busseq_simulate(int B, const int* n_b, int G, int K, uint32_t data_seed,
+                              double g0_lo, double g0_hi, int32_t* Y, int32_t* W_true,
+                              int32_t* X_true);
+
+/* message of the last failing call on this thread */
+const char* busseq_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
