@@ -1,0 +1,176 @@
+"""Benchmark of the BUSseq Gibbs sweep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--config c2] [--precision 32]
+
+One "step" is one full Gibbs sweep (W, pi, X/Z, alpha, gamma, L, p/tau0,
+beta, nu, delta, phi) of the plain form: one chain on the synthetic data set
+of the chosen configuration, counts and state resident in HBM.  Prints ONE JSON
+line on stdout; diagnostics go to stderr.
+
+Configurations (synthetic counts from the BUSseq generative model):
+  c1  B=4, n_b=300/300/200/200 (N=1000), G=3000, K=5
+  c2  B=4, n_b=8000/6000/4000/2000 (N=20000), G=3000, K=7   [default]
+  c3  B=8, n_b=40k/30k/25k/20k/15k/10k/6k/4k (N=150000), G=5000, K=12, gamma0~U(-1,0.5)
+
+The roofline leg times every kernel of the sweep with HIP events on the
+sampler's own stream and reports the dominant one against HBM bandwidth.  The
+cpu_baseline leg times the CPU oracle (OpenMP) on a bounded number of sweeps of
+the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    "c1": dict(n_b=[300, 300, 200, 200], G=3000, K=5, data_seed=1001, g0=(-2.0, 0.0)),
+    "c2": dict(n_b=[8000, 6000, 4000, 2000], G=3000, K=7, data_seed=1002, g0=(-2.0, 0.0)),
+    "c3": dict(n_b=[40000, 30000, 25000, 20000, 15000, 10000, 6000, 4000], G=5000, K=12,
+               data_seed=1003, g0=(-1.0, 0.5)),
+}
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gibbs sweeps/sec and cell×gene updates/s at N=20k,G=3k,K=7; batched vs plain"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, N, G, n_zero):
+    """Algorithmic HBM bytes of one launch: the int32 state matrix S read once
+    (4 B per cell x gene); k_xz_alpha also writes S back for the Y = 0 entries."""
+    base = 4.0 * N * G
+    if name == "k_xz_alpha":
+        return base + 4.0 * n_zero
+    if name in ("k_w", "k_beta_nu", "k_delta", "k_phi"):
+        return base
+    return None
+
+
+def pmc_traffic(kernel, cfg_name):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{cfg_name}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("kernels", {}).get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(Y, cfg, precision, budget_s):
+    from oracle import pyoracle as po
+    threads = min(16, os.cpu_count() or 1)
+    o = po.Oracle(Y, cfg["n_b"], cfg["K"], seed=13579, precision=precision, n_iter=4000, n_burnin=2000,
+                  n_unchanged=500, threads=threads)
+    t0 = time.perf_counter()
+    o.sweeps(1)
+    one = time.perf_counter() - t0
+    n = max(1, min(50, int(budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.sweeps(n)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": n / dt, "unit": "Gibbs sweeps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} sweeps of the same workload after 1 warm-up sweep, CPU oracle "
+                      f"(oracle/oracle.c, OpenMP, {threads} threads, precision {precision})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
+    ap.add_argument("--kernel-sweeps", type=int, default=5)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(chains, K-sweeps and data sets shard across GPUs without communication).", file=sys.stderr)
+        sys.exit(2)
+
+    import busseq_gpu_amd as bq
+
+    cfg = CONFIGS[args.config]
+    n_b, G, K = cfg["n_b"], cfg["G"], cfg["K"]
+    N = sum(n_b)
+    t0 = time.perf_counter()
+    Y = bq.simulate(n_b, G, K, cfg["data_seed"], gamma0_range=cfg["g0"])
+    n_zero = int((Y == 0).sum())
+    log(f"[bench] data {args.config}: N={N} G={G} K={K} zeros={n_zero / Y.size:.3f} ({time.perf_counter() - t0:.1f}s)")
+
+    calib_ms = bq.copy_probe(1 << 30)
+    log(f"[bench] copy probe 1 GiB: {calib_ms:.3f} ms = {2 * (1 << 30) / calib_ms / 1e6:.0f} GB/s (read+write)")
+
+    t0 = time.perf_counter()
+    s = bq.Sampler(Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500, precision=args.precision)
+    log(f"[bench] init {time.perf_counter() - t0:.1f}s")
+    s.sweeps(args.warmup)
+    w0 = time.perf_counter()
+    ms = s.time_sweeps(args.steps)
+    wall = time.perf_counter() - w0
+    sweeps_per_s = args.steps / (ms / 1e3)
+    log(f"[bench] {args.steps} sweeps: {ms:.2f} ms (events), wall {wall * 1e3:.2f} ms -> {sweeps_per_s:.1f} sweeps/s")
+
+    kt = s.kernel_times(args.kernel_sweeps)
+    for k, v in kt.items():
+        log(f"[bench]   {k:14s} {v * 1e3:9.1f} us")
+    dom = max(kt, key=kt.get)
+    dom_bytes = kernel_bytes(dom, N, G, n_zero)
+    roof = None
+    if dom_bytes is not None:
+        ach = dom_bytes / (kt[dom] * 1e-3) / 1e9
+        traffic = pmc_traffic(dom, args.config)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                "traffic": traffic, "algorithmic_bytes": dom_bytes, "avg_ms": round(kt[dom], 5)}
+    sweep_bytes = 16.0 * N * G
+    s.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(Y, cfg, args.precision, args.cpu_budget)
+        log(f"[bench] cpu oracle: {cpu['value']:.4f} sweeps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(sweeps_per_s, 3),
+        "unit": "Gibbs sweeps/s",
+        "cell_gene_updates_per_s": round(sweeps_per_s * N * G, 1),
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == 32 else "f64",
+        "data": "synthetic (BUSseq generative model, seeded)",
+        "config": {"workload": f"plain BUSseq Gibbs sweep, one chain, {args.config}: N={N} "
+                               f"({'/'.join(map(str, n_b))}), G={G}, K={K}, B={len(n_b)}",
+                   "chains": 1, "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": roof,
+        "sweep_roofline": {"bytes_per_sweep": sweep_bytes,
+                           "achieved": round(sweep_bytes * sweeps_per_s / 1e9, 2),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(sweep_bytes * sweeps_per_s / 1e9 / HBM_PEAK_GBS, 5)},
+        "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": round(sweeps_per_s / cpu["value"], 1) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -13,7 +13,7 @@ import os as _os
 
 import numpy as _np
 
-__all__ = ["Sampler", "simulate", "run", "read_counts", "read_dims", "write_outputs",
+__all__ = ["Sampler", "simulate", "run", "read_counts", "read_dims", "kernel_names", "copy_probe",
            "library_path", "BusseqError", "STATUS"]
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
@@ -85,6 +85,11 @@ def _load():
     lib.busseq_result_free.restype = None
     lib.busseq_read_dims.argtypes = [_C.c_char_p, _C.c_int, _C.POINTER(_C.c_int)]
     lib.busseq_read_counts.argtypes = [_C.c_char_p, _C.c_int, _C.c_int, _PI]
+    lib.busseq_kernel_count.restype = _C.c_int
+    lib.busseq_kernel_name.argtypes = [_C.c_int]
+    lib.busseq_kernel_name.restype = _C.c_char_p
+    lib.busseq_kernel_times.argtypes = [vp, _C.c_int, _PD]
+    lib.busseq_copy_probe.argtypes = [_C.c_size_t, _C.c_int, _C.POINTER(_C.c_double)]
     lib.busseq_simulate.argtypes = [_C.c_int, _C.POINTER(_C.c_int), _C.c_int, _C.c_int, _C.c_uint32,
                                     _C.c_double, _C.c_double, _PI, _PI, _PI]
     _lib = lib
@@ -131,6 +136,19 @@ def simulate(n_b, G, K, data_seed, gamma0_range=(-2.0, 0.0), want_truth=False):
     _check(lib.busseq_simulate(len(n_b), nb, int(G), int(K), int(data_seed) & 0xFFFFFFFF,
                                float(gamma0_range[0]), float(gamma0_range[1]), _ip(Y), _ip(W), _ip(X)))
     return (Y, W, X) if want_truth else Y
+
+
+def kernel_names():
+    lib = _load()
+    return [lib.busseq_kernel_name(j).decode() for j in range(lib.busseq_kernel_count())]
+
+
+def copy_probe(nbytes, device=0):
+    """One streaming device copy of nbytes; returns milliseconds."""
+    lib = _load()
+    ms = _C.c_double()
+    _check(lib.busseq_copy_probe(int(nbytes), int(device), _C.byref(ms)))
+    return ms.value
 
 
 def read_dims(path, B):
@@ -221,6 +239,13 @@ class Sampler:
         _check(self._lib.busseq_time_sweeps(self._h, int(n), _C.byref(ms)))
         return ms.value
 
+    def kernel_times(self, n):
+        """Average milliseconds per kernel over n eagerly launched sweeps."""
+        names = kernel_names()
+        ms = _np.zeros(len(names))
+        _check(self._lib.busseq_kernel_times(self._h, int(n), _dp(ms)))
+        return dict(zip(names, ms.tolist()))
+
     def state(self, chain=0):
         N, G, K, B = self.N, self.G, self.K, self.B
         s = {"X": _np.zeros((N, G), _np.int32), "Z": _np.zeros((N, G), _np.int32),
@@ -270,6 +295,3 @@ def run(Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500, keep
     finally:
         lib.busseq_result_free(_C.byref(r))
 
-
-def write_outputs(result_handle, prefix, print_preserved=False):
-    raise NotImplementedError("use run(..., prefix=...) or the BUSseq_gpu executable")

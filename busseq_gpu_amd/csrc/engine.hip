@@ -852,8 +852,14 @@ struct busseq_sampler {
 
 namespace {
 
+static const char* const kKernelNames[] = {
+    "k_prep", "k_w", "k_plan", "k_xz_alpha", "k_alpha_dec", "k_gamma_ptau", "k_beta_nu",
+    "k_bn_dec", "k_delta", "k_phi", "k_phi_dec", "k_record", "k_tick"};
+static const int kNumKernels = 13;
+
+// ev: optional array of kNumKernels + 1 events recorded around each launch
 template <typename T>
-busseq_status enqueue_sweep(busseq_sampler* s)
+busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
 {
     const dim3 blk(256);
     const int C = s->C;
@@ -875,19 +881,23 @@ busseq_status enqueue_sweep(busseq_sampler* s)
     const dim3 g_rec((rec + 255) / 256, 1, C);
     hipStream_t st = s->stream;
     const ChainDev* cd = s->d_cd;
-    hipLaunchKernelGGL(k_prep<T>, g_prep, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_w<T>, g_rows, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_plan, g_plan, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_xz_alpha<T>, g_tile, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_alpha_dec<T>, g_gene, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_beta_nu<T>, g_tile, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_bn_dec<T>, g_gene, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_phi_dec<T>, g_gene, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_record, g_rec, blk, 0, st, cd);
-    hipLaunchKernelGGL(k_tick, dim3(1, 1, C), dim3(64), 0, st, cd);
+    int q = 0;
+#define EV() do { if (ev) HIPCHK(hipEventRecord(ev[q], st)); ++q; } while (0)
+    EV(); hipLaunchKernelGGL(k_prep<T>, g_prep, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_w<T>, g_rows, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_plan, g_plan, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_xz_alpha<T>, g_tile, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_gene, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_beta_nu<T>, g_tile, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_bn_dec<T>, g_gene, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_phi_dec<T>, g_gene, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_record, g_rec, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_tick, dim3(1, 1, C), dim3(64), 0, st, cd);
+    EV();
+#undef EV
     HIPCHK(hipGetLastError());
     return BUSSEQ_OK;
 }
@@ -899,7 +909,7 @@ busseq_status build_graph(busseq_sampler* s)
     busseq_status rc = s->prec == 32 ? enqueue_sweep<float>(s) : enqueue_sweep<double>(s);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
-    if (rc != BUSSEQ_OK) { if (g) hipGraphDestroy(g); return rc; }
+    if (rc != BUSSEQ_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) { bq_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return BUSSEQ_ERR_DEVICE; }
     s->graph = g;
     HIPCHK(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
@@ -1108,13 +1118,13 @@ extern "C" busseq_status busseq_create(const busseq_config* cfg, const int32_t* 
 extern "C" void busseq_destroy(busseq_sampler* s)
 {
     if (!s) return;
-    hipSetDevice(s->dev);
-    if (s->stream) hipStreamSynchronize(s->stream);
-    if (s->gexec) hipGraphExecDestroy(s->gexec);
-    if (s->graph) hipGraphDestroy(s->graph);
-    for (void* p : s->allocs) hipFree(p);
-    if (s->d_cd) hipFree(s->d_cd);
-    if (s->stream) hipStreamDestroy(s->stream);
+    (void)hipSetDevice(s->dev);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->d_cd) (void)hipFree(s->d_cd);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
@@ -1148,10 +1158,75 @@ extern "C" busseq_status busseq_time_sweeps(busseq_sampler* s, int n, double* ms
     HIPCHK(hipEventSynchronize(e1));
     float f = 0.f;
     HIPCHK(hipEventElapsedTime(&f, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     *ms = f;
     s->iter += n;
+    return BUSSEQ_OK;
+}
+
+extern "C" int busseq_kernel_count(void) { return kNumKernels; }
+
+extern "C" const char* busseq_kernel_name(int j)
+{
+    return (j >= 0 && j < kNumKernels) ? kKernelNames[j] : "";
+}
+
+extern "C" busseq_status busseq_kernel_times(busseq_sampler* s, int n, double* ms)
+{
+    if (!s || n < 1 || !ms) { bq_set_error("bad arguments to busseq_kernel_times"); return BUSSEQ_ERR_ARG; }
+    HIPCHK(hipSetDevice(s->dev));
+    std::vector<hipEvent_t> ev(kNumKernels + 1);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    for (int j = 0; j < kNumKernels; ++j) ms[j] = 0.0;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int it = 0; it < n; ++it) {
+        busseq_status rc = s->prec == 32 ? enqueue_sweep<float>(s, ev.data()) : enqueue_sweep<double>(s, ev.data());
+        if (rc != BUSSEQ_OK) return rc;
+        HIPCHK(hipEventSynchronize(ev[kNumKernels]));
+        for (int j = 0; j < kNumKernels; ++j) {
+            float f = 0.f;
+            HIPCHK(hipEventElapsedTime(&f, ev[j], ev[j + 1]));
+            ms[j] += f;
+        }
+    }
+    for (int j = 0; j < kNumKernels; ++j) ms[j] /= n;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    s->iter += n;
+    return BUSSEQ_OK;
+}
+
+// streaming copy of known size; calibrates the HBM byte counters of a profiler
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+{
+    for (size_t j = (size_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (size_t)gridDim.x * 256) dst[j] = src[j];
+}
+
+extern "C" busseq_status busseq_copy_probe(size_t bytes, int device, double* ms)
+{
+    if (bytes < 16 || !ms) { bq_set_error("bad arguments to busseq_copy_probe"); return BUSSEQ_ERR_ARG; }
+    HIPCHK(hipSetDevice(device));
+    size_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n * 16));
+    HIPCHK(hipMalloc(&b, n * 16));
+    HIPCHK(hipMemset(a, 1, n * 16));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipEventRecord(e0, nullptr));
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, nullptr, (const uint4*)a, (uint4*)b, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipFree(a));
+    HIPCHK(hipFree(b));
     return BUSSEQ_OK;
 }
 
@@ -1296,8 +1371,8 @@ extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result*
         std::vector<double> ll((size_t)N * K);
         HIPCHK(hipStreamSynchronize(s->stream));
         HIPCHK(hipMemcpy(ll.data(), d_ll, ll.size() * 8, hipMemcpyDeviceToHost));
-        hipFree(d_pm);
-        hipFree(d_ll);
+        (void)hipFree(d_pm);
+        (void)hipFree(d_ll);
         std::vector<int> batch(N);
         { int i = 0; for (int b = 0; b < B; ++b) for (int q = 0; q < s->n_b[b]; ++q) batch[i++] = b; }
         double tot = 0.0;

@@ -19,6 +19,7 @@
 #ifndef BUSSEQ_H
 #define BUSSEQ_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -112,6 +113,16 @@ busseq_status busseq_read_counts(const char* path, int N, int G, int32_t* Y);
 busseq_status busseq_simulate(int B, const int* n_b, int G, int K, uint32_t data_seed,
                               double g0_lo, double g0_hi, int32_t* Y, int32_t* W_true,
                               int32_t* X_true);
+
+/* Measurement helpers (no counterpart in the reference, which has no timers).
+ * busseq_kernel_times runs n sweeps as individual launches with HIP events
+ * between them and returns the average milliseconds of each of the
+ * busseq_kernel_count() kernels (names from busseq_kernel_name). */
+int busseq_kernel_count(void);
+const char* busseq_kernel_name(int j);
+busseq_status busseq_kernel_times(busseq_sampler* s, int n, double* ms);
+/* one streaming copy of `bytes` bytes on the device (profiler counter calibration) */
+busseq_status busseq_copy_probe(size_t bytes, int device, double* ms);
 
 /* message of the last failing call on this thread */
 const char* busseq_last_error(void);

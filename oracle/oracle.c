@@ -354,6 +354,25 @@ static void build_tiles(orc_t* o)
     free(off); free(cur);
 }
 
+/*
+ * X/Z update of one entry with Y = 0 (cu:484-512).  s is the encoded state:
+ * 0 = (X=0, Z=0), ~x = (X=x, Z=1).  If X = 0, Z ~ Bern(logistic(gamma0)),
+ * i.e. Z = 1 when u > 1/(1+e^gamma0) = qz; if Z = 1 an NB(mu, phi)
+ * independence proposal x' is accepted when
+ *   log u <= log(1+e^{-g0-g1 x}) - log(1+e^{-g0-g1 x'})   (log form of cu:504);
+ * if Z = 0 then X = 0.  Returns the new encoded state.
+ */
+static int32_t xz_update(int32_t s, double mu, double phi, double g0, double g1, double qz, bq_rng* r)
+{
+    int xo = s < 0 ? ~s : 0;
+    int z = xo == 0 ? (bq_uniform(r) > qz) : 1;
+    if (!z) return 0;
+    int xp = bq_rnbinom(r, mu, phi);
+    double lr = bq_neg_logit_sp(g0, g1, (double)xo) - bq_neg_logit_sp(g0, g1, (double)xp);
+    int xi = bq_log(bq_uniform(r)) <= lr ? xp : xo;
+    return ~xi;
+}
+
 #define DEFINE_PASSES(T, SFX)                                                              \
 /* subtype draw: categorical full conditional of W_i (replaces MH cu:820-854) */           \
 static void pass_w_##SFX(orc_t* o, int t)                                                  \
@@ -432,15 +451,10 @@ static void pass_xz_alpha_##SFX(orc_t* o, int t)                                
                 int xi;                                                                    \
                 if (s <= 0) {                                                              \
                     bq_rng r; bq_rng_init(&r, o->seed, o->chain, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ); \
-                    int xo = s < 0 ? ~s : 0;                                               \
-                    int z = xo == 0 ? (bq_uniform(&r) > qz[b]) : 1;                        \
-                    if (z) {                                                               \
-                        int xp = bq_rnbinom(&r, (double)mu, o->phi[bg]);                   \
-                        double lr = bq_neg_logit_sp(g0, g1, (double)xo) - bq_neg_logit_sp(g0, g1, (double)xp); \
-                        xi = bq_log(bq_uniform(&r)) <= lr ? xp : xo;                       \
-                        o->S[e] = ~xi;                                                     \
-                        sm[0 * B + b] += 1; sm[1 * B + b] += (uint64_t)xi;                 \
-                    } else { xi = 0; o->S[e] = 0; }                                        \
+                    int32_t sn = xz_update(s, (double)mu, o->phi[bg], g0, g1, qz[b], &r);  \
+                    o->S[e] = sn;                                                          \
+                    xi = sn < 0 ? ~sn : 0;                                                 \
+                    if (sn < 0) { sm[0 * B + b] += 1; sm[1 * B + b] += (uint64_t)xi; }    \
                     if (xi < H) hd[(size_t)b * H + xi] += 1;                               \
                 } else xi = s;                                                             \
                 if (xi >= H) {                                                             \
@@ -1015,13 +1029,35 @@ void orc_math(int fn, const double* x, const double* y, double* out, int n)
     }
 }
 
+/* runs the X/Z update of one Y = 0 entry for n steps with fixed parameters;
+ * counts[x*2 + z] for x < nx (the stationary law test) */
+void orc_xz_chain(double mu, double phi, double g0, double g1, uint32_t seed, int n, int nx, int64_t* counts)
+{
+    int32_t s = 0;
+    double qz = 1.0 / (1.0 + bq_exp(g0));
+    for (int t = 0; t < n; ++t) {
+        bq_rng r; bq_rng_init(&r, seed, 7u, 0u, 0u, (uint32_t)t, BQ_PH_XZ);
+        s = xz_update(s, mu, phi, g0, g1, qz, &r);
+        int x = s < 0 ? ~s : 0, z = s < 0;
+        if (x < nx) counts[x * 2 + z] += 1;
+    }
+}
+
 /* element terms for the tests: fn 0 tmh(x,phi,r,dlt,c), 1 tw(x,phi,r0,beta,em),
- * 2 tphi(x,phi,phip,mu) with the per-(b,g) constants derived here, 3 ratio(mu,phi) */
+ * 2 tphi(x,phi,phip,mu) with the per-(b,g) constants derived here, 3 ratio(mu,phi),
+ * 4 lr_gamma1(g,gn), 5 lr_phi(phi,phin), 6 logodds_L(p,tau0,beta), 7 lr_normal(old,new,m,s2),
+ * 8 bic_elem(a,bk,nb,dl,g0,g1,phi,y), 9 lr_beta(old,new,L,tau0) */
 void orc_terms(int fn, int prec, const double* a, int na, double* out, int n)
 {
     for (int j = 0; j < n; ++j) {
         const double* v = a + (size_t)j * na;
         double res = 0.0;
+        if (fn == 4) { out[j] = bq_lr_gamma1(v[0], v[1]); continue; }
+        if (fn == 5) { out[j] = bq_lr_phi(v[0], v[1]); continue; }
+        if (fn == 6) { out[j] = bq_logodds_L(v[0], v[1], v[2]); continue; }
+        if (fn == 7) { out[j] = bq_lr_normal(v[0], v[1], v[2], v[3]); continue; }
+        if (fn == 8) { out[j] = bq_bic_elem(v[0], v[1], v[2], v[3], v[4], v[5], v[6], (int)v[7]); continue; }
+        if (fn == 9) { out[j] = bq_lr_beta(v[0], v[1], (int)v[2], v[3]); continue; }
         if (fn == 2) {
             double f = v[1], fp = v[2], d = fp - f;
             if (prec == 32)

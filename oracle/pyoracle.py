@@ -48,6 +48,7 @@ def load():
     lib.orc_stream.argtypes = [C.c_uint32] * 6 + [C.c_int, C.c_double, C.c_double, C.c_int, _P(np.float64)]
     lib.orc_math.argtypes = [C.c_int, _P(np.float64), _P(np.float64), _P(np.float64), C.c_int]
     lib.orc_terms.argtypes = [C.c_int, C.c_int, _P(np.float64), C.c_int, _P(np.float64), C.c_int]
+    lib.orc_xz_chain.argtypes = [C.c_double] * 4 + [C.c_uint32, C.c_int, C.c_int, _P(np.int64)]
     _lib = lib
     return lib
 
@@ -90,11 +91,22 @@ def philox(ctr, key):
     return out
 
 
+def xz_chain(mu, phi, g0, g1, seed, n, nx):
+    lib = load()
+    out = np.zeros(nx * 2, np.int64)
+    lib.orc_xz_chain(mu, phi, g0, g1, seed, n, nx, out)
+    return out.reshape(nx, 2)
+
+
+TERMS = {"tmh": 0, "tw": 1, "tphi": 2, "ratio": 3, "lr_gamma1": 4, "lr_phi": 5, "logodds_L": 6,
+         "lr_normal": 7, "bic_elem": 8, "lr_beta": 9}
+
+
 def terms(fn, prec, args):
     lib = load()
     a = np.ascontiguousarray(args, np.float64)
     out = np.zeros(a.shape[0])
-    lib.orc_terms(fn, prec, a.ravel(), a.shape[1], out, a.shape[0])
+    lib.orc_terms(TERMS.get(fn, fn), prec, a.ravel(), a.shape[1], out, a.shape[0])
     return out
 
 

@@ -1,0 +1,190 @@
+"""Drop-in boundary: C ABI exports, input readers, output writers and the
+BUSseq_gpu command line (BUSseq_gpu.cu:1026-1161, 1610-2018; README.md:17-49)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import busseq_gpu_amd as bq
+from busseq_gpu_amd import _Result, _load
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "busseq_gpu_amd", "bin", "BUSseq_gpu")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared(header, prefix):
+    txt = open(os.path.join(ROOT, header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(" + prefix + r"\w+)\s*\(", txt)))
+
+
+def exported(so):
+    out = subprocess.run(["nm", "-D", "--defined-only", so], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_library_exports_every_declared_symbol():
+    syms = declared("include/busseq.h", "busseq_")
+    assert len(syms) >= 20
+    missing = set(syms) - exported(bq.library_path())
+    assert not missing, missing
+    _load()  # loads without a GPU
+
+
+def test_oracle_exports_every_declared_symbol():
+    syms = declared("oracle/oracle.h", "orc_")
+    missing = set(syms) - exported(os.path.join(ROOT, "oracle", "liboracle.so"))
+    assert not missing, missing
+
+
+def test_read_dims_reference_fixture():
+    # count_data/demo_dim.txt of the reference (-N file)
+    assert bq.read_dims(os.path.join(GOLD, "demo_dim.txt"), 4) == [300, 300, 200, 200]
+
+
+def test_read_dims_errors(tmp_path):
+    p = tmp_path / "d.txt"
+    p.write_text("300 300")
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_dims(str(p), 4)
+    assert e.value.status == 3
+    p.write_text("300 x 1 2")
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_dims(str(p), 4)
+    assert e.value.status == 3
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_dims(str(tmp_path / "missing.txt"), 4)
+    assert e.value.status == 2
+
+
+def test_read_counts_roundtrip_and_errors(tmp_path):
+    rng = np.random.default_rng(0)
+    Y = rng.integers(0, 50, size=(7, 5)).astype(np.int32)
+    p = tmp_path / "c.txt"
+    p.write_text("\n".join("\t".join(str(v) for v in row) for row in Y) + "\n")
+    np.testing.assert_array_equal(bq.read_counts(str(p), 7, 5), Y)
+    p.write_text(" ".join(str(v) for v in Y.ravel()))          # any whitespace layout
+    np.testing.assert_array_equal(bq.read_counts(str(p), 7, 5), Y)
+    p.write_text(" ".join(str(v) for v in Y.ravel()[:-1]))     # short
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), 7, 5)
+    assert e.value.status == 3 and "found 34" in str(e.value)
+    p.write_text("1 2 3.5 " + " ".join("1" for _ in range(40)))  # not an integer
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), 7, 5)
+    assert e.value.status == 3 and "3.5" in str(e.value)
+    p.write_text("-1 " + " ".join("1" for _ in range(40)))       # negative count
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), 7, 5)
+    assert e.value.status == 3
+
+
+def fmt(v):
+    return "%f" % v
+
+
+def make_result(B=2, N=3, G=4, K=3, P=2, seed=3):
+    rng = np.random.default_rng(seed)
+    keep = {}
+
+    def arr(shape, dt):
+        a = np.ascontiguousarray(rng.normal(size=shape) * 3 if dt == np.float64 else rng.integers(0, 9, size=shape),
+                                 dtype=dt)
+        keep[id(a)] = a
+        return a
+
+    d = {"X": arr((N, G), np.int32), "Z": arr((N, G), np.int32), "W": arr(N, np.int32),
+         "gamma": arr((2, B), np.float64), "alpha": arr(G, np.float64), "L": arr((K, G), np.float64),
+         "beta": arr((K, G), np.float64), "ptau": arr(2, np.float64), "nu": arr((B, G), np.float64),
+         "delta": arr(N, np.float64), "phi": arr((B, G), np.float64), "pi": arr((B, K), np.float64),
+         "tr_gamma": arr((2 * B, P), np.float64), "tr_alpha": arr((G, P), np.float64),
+         "tr_L": arr(((K - 1) * G, P), np.int32), "tr_beta": arr(((K - 1) * G, P), np.float64),
+         "tr_ptau": arr((2, P), np.float64), "tr_nu": arr(((B - 1) * G, P), np.float64),
+         "tr_delta": arr((N, P), np.float64), "tr_phi": arr((B * G, P), np.float64),
+         "tr_W": arr((N, P), np.int32), "tr_pi": arr((B * K, P), np.float64)}
+    r = _Result()
+    r.B, r.N, r.G, r.K, r.n_pres = B, N, G, K, P
+    for k, a in d.items():
+        ptr = a.ctypes.data_as(C.POINTER(C.c_int32 if a.dtype == np.int32 else C.c_double))
+        setattr(r, k, ptr)
+    r.bic = 12345.678901234
+    return r, d
+
+
+def mat(a, f=fmt):
+    a = np.atleast_2d(a)
+    return "".join("\t".join(f(v) for v in row) + "\n" for row in a)
+
+
+def test_writers_match_reference_formats(tmp_path):
+    lib = _load()
+    r, d = make_result()
+    prefix = str(tmp_path / "out")
+    assert lib.busseq_write_outputs(C.byref(r), prefix.encode(), 1) == 0
+    rd = lambda name: open(prefix + name).read()
+    i = lambda v: "%d" % v
+    B, N, G, K = 2, 3, 4, 3
+    assert rd("_post_Z.txt") == mat(d["Z"], i)
+    assert rd("_post_X.txt") == mat(d["X"], i)
+    assert rd("_post_gamma.txt") == mat(d["gamma"])
+    assert rd("_post_alpha.txt") == mat(d["alpha"][:, None])
+    assert rd("_post_L.txt") == mat(d["L"][1:])
+    assert rd("_post_beta.txt") == "\t".join(["0.000000"] * G) + "\n" + mat(d["beta"][1:])
+    assert rd("_post_p.txt") == fmt(d["ptau"][0]) + "\t" + fmt(d["ptau"][1]) + "\t"   # no newline (cu:1725-1727)
+    assert rd("_post_nu.txt") == "\t".join(["0.000000"] * G) + "\n" + mat(d["nu"][1:])
+    assert rd("_post_delta.txt") == mat(d["delta"][:, None])
+    assert rd("_post_phi.txt") == mat(d["phi"])
+    assert rd("_post_W.txt") == mat(d["W"][:, None], i)
+    assert rd("_post_pi.txt") == mat(d["pi"])
+    assert rd("_BIC.txt") == "12345.678901\n"
+    for name, key, f in (("gamma", "tr_gamma", fmt), ("alpha", "tr_alpha", fmt), ("L", "tr_L", i),
+                         ("beta", "tr_beta", fmt), ("p", "tr_ptau", fmt), ("nu", "tr_nu", fmt),
+                         ("delta", "tr_delta", fmt), ("phi", "tr_phi", fmt), ("W", "tr_W", i), ("pi", "tr_pi", fmt)):
+        assert rd(f"_preserved_{name}.txt") == mat(d[key], f), name
+
+
+def test_writer_reports_unwritable_prefix(tmp_path):
+    lib = _load()
+    r, d = make_result()
+    assert lib.busseq_write_outputs(C.byref(r), str(tmp_path / "no" / "such" / "dir").encode(), 0) == 2
+
+
+def cli(*args, cwd=None):
+    return subprocess.run([EXE, *args], capture_output=True, text=True, cwd=cwd, timeout=120)
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["-B", "1", "-s", "1", "-i", "10", "-b", "5"], "The batch number must be greater than one."),
+    (["-B", "4", "-s", "1", "-i", "10", "-b", "20"], "Burn-in iterations must be less than total number of iterations."),
+    (["-B", "4", "-K", "301", "-s", "1", "-i", "10", "-b", "5"],
+     "The sample size in any batch must be greater than the assumed cell type number."),
+])
+def test_cli_validation_messages(args, msg):
+    p = cli(*args)
+    assert p.returncode == 255, p.stdout + p.stderr
+    assert msg in p.stdout
+    assert "Seed is 1" in p.stdout
+
+
+def test_cli_unknown_option_prints_usage():
+    p = cli("-z")
+    assert p.returncode == 1
+    assert "Error Usage:" in p.stdout
+
+
+def test_cli_missing_count_file(tmp_path):
+    p = cli("-B", "4", "-N", os.path.join(GOLD, "demo_dim.txt"), "-G", "10", "-K", "3", "-s", "3",
+            "-c", str(tmp_path / "nope.txt"), "-i", "10", "-b", "5", "-o", str(tmp_path / "o"))
+    assert p.returncode == 1
+    assert "Start reading file." in p.stdout and "cannot open" in p.stdout
+    assert not list(tmp_path.glob("o_*"))
+
+
+def test_cli_derived_defaults(tmp_path):
+    # without -b, burn-in = n_iter/2 (cu:1101-1103): -i 10 gives burn-in 5 < 10, valid until the count file
+    p = cli("-B", "2", "-s", "4", "-i", "10", "-c", str(tmp_path / "none.txt"))
+    assert "Burn-in" not in p.stdout and p.returncode == 1

@@ -1,0 +1,122 @@
+"""GPU parity: the gfx950 sampler against the CPU oracle, sweep by sweep.
+
+Both are driven by the same Philox streams and the same numerics
+(include/busseq/numerics.h, model.h) and use the same reduction grouping, so
+integer draws (X, Z, W, L) must match bit for bit and the continuous
+parameters must be identical (rtol 1e-12 written below; equality is expected).
+"""
+import numpy as np
+import pytest
+
+import busseq_gpu_amd as bq
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+INT_KEYS = ("X", "Z", "W", "L")
+FLOAT_KEYS = ("alpha", "beta", "nu", "delta", "phi", "gamma", "pi", "ptau")
+
+
+def compare(gs, os_, where):
+    for k in INT_KEYS:
+        a, b = gs[k], os_[k]
+        bad = np.argwhere(a != b)
+        assert bad.size == 0, (f"{where}: {k} differs at {bad[:5].tolist()} "
+                               f"gpu={a[tuple(bad[0])]} oracle={b[tuple(bad[0])]}")
+    for k in FLOAT_KEYS:
+        np.testing.assert_allclose(gs[k], os_[k], rtol=1e-12, atol=1e-300, err_msg=f"{where}: {k}")
+
+
+def run_pair(Y, n_b, K, steps, prec=32, tile_rows=64, **extra):
+    kw = dict(seed=extra.pop("seed", 13579), n_iter=extra.pop("n_iter", 40), n_burnin=extra.pop("n_burnin", 10),
+              n_unchanged=extra.pop("n_unchanged", 5), precision=prec)
+    print(f"[parity] prec={prec} N={sum(n_b)} G={Y.shape[1]} K={K} B={len(n_b)} TR={tile_rows}", flush=True)
+    o = po.Oracle(Y, n_b, K, tile_rows=tile_rows, **kw)
+    s = bq.Sampler(Y, n_b, K, tile_rows=tile_rows, **kw)
+    compare(s.state(), o.state(), "init")
+    for step in steps:
+        s.sweeps(step)
+        o.sweeps(step)
+        compare(s.state(), o.state(), f"iter {o.iteration}")
+    return s, o
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_sweeps_match_oracle(small_data, prec):
+    d = small_data
+    s, o = run_pair(d["Y"], d["n_b"], d["K"], (1, 1, 3, 10), prec)
+    s.close()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_posterior_and_bic_match_oracle(small_data, prec):
+    d = small_data
+    s, o = run_pair(d["Y"], d["n_b"], d["K"], (40,), prec, n_iter=40, n_burnin=25, n_unchanged=5)
+    r = s.finish()
+    p = o.posterior()
+    for k in ("gamma", "alpha", "L", "beta", "ptau", "nu", "delta", "phi", "pi"):
+        np.testing.assert_allclose(r[k], p[k], rtol=1e-12, atol=1e-300, err_msg=k)
+    np.testing.assert_array_equal(r["W"], p["W"])
+    np.testing.assert_allclose(r["bic"], p["bic"], rtol=1e-12)
+    s.close()
+
+
+def test_tile_rows_and_k2_b2():
+    n_b = [9, 13]
+    Y = bq.simulate(n_b, 70, 2, 31)
+    s, o = run_pair(Y, n_b, 2, (1, 4, 5), 32, tile_rows=4)
+    s.close()
+
+
+def test_large_counts_overflow_histogram():
+    # counts >= 1024 take the direct gamma-term path instead of the histogram
+    n_b = [20, 20]
+    Y = bq.simulate(n_b, 96, 3, 5)
+    Y[::3, ::7] = Y[::3, ::7] * 400 + 1500
+    s, o = run_pair(Y, n_b, 3, (1, 2, 4), 32)
+    s.close()
+
+
+def test_edge_batch_equals_k_and_zero_gene():
+    n_b = [4, 4, 4]
+    Y = bq.simulate(n_b, 65, 4, 11)
+    Y[:, 3] = 0    # a gene that is never observed
+    s, o = run_pair(Y, n_b, 4, (1, 5), 64)
+    s.close()
+
+
+def test_batched_chains_equal_plain_chains(small_data):
+    d = small_data
+    kw = dict(seed=99, n_iter=30, n_burnin=10, n_unchanged=5, precision=32)
+    with bq.Sampler(d["Y"], d["n_b"], d["K"], chains=3, **kw) as sb:
+        sb.sweeps(12)
+        for c in range(3):
+            with bq.Sampler(d["Y"], d["n_b"], d["K"], chains=1, chain0=c, **kw) as sp:
+                sp.sweeps(12)
+                compare(sb.state(c), sp.state(), f"chain {c}")
+
+
+def test_set_state_roundtrip(small_data):
+    d = small_data
+    kw = dict(seed=5, n_iter=30, n_burnin=10, n_unchanged=5, precision=32)
+    o = po.Oracle(d["Y"], d["n_b"], d["K"], **kw)
+    o.sweeps(3)
+    st = o.state()
+    with bq.Sampler(d["Y"], d["n_b"], d["K"], **kw) as s:
+        s.sweeps(3)
+        compare(s.state(), st, "iter 3")
+        # perturb, then restore the oracle state and continue both
+        s.set_state({"alpha": st["alpha"] + 0.5})
+        s.set_state({k: st[k] for k in ("X", "Z", "W", "alpha", "beta", "L", "nu", "delta", "phi",
+                                         "gamma", "pi", "ptau")})
+        s.sweeps(2)
+        o.sweeps(2)
+        compare(s.state(), o.state(), "after set_state")
+
+
+@pytest.mark.slow
+def test_c1_shape_parity():
+    n_b = [300, 300, 200, 200]
+    Y = bq.simulate(n_b, 3000, 5, 1001)
+    s, o = run_pair(Y, n_b, 5, (1, 2), 32, n_iter=4000, n_burnin=2000, n_unchanged=500)
+    s.close()
