@@ -12,6 +12,7 @@
 #define BQ_KMAX 16     // largest K supported (register arrays in the row kernels)
 #define BQ_BMAX 128    // largest B (the reference's __constant__ limit, cu:9-10)
 #define BQ_WAVE 64
+#define BQ_CHUNK 1024 // genes per row-sum chunk (part of the reduction contract)
 
 struct ChainDev {
     // dimensions and schedule
@@ -25,6 +26,10 @@ struct ChainDev {
     const int32_t* batch;          // [N] batch of each cell
     const int32_t* boff;           // [B+1] first cell of each batch
     const int64_t* hist_s;         // [B*H] static histogram of Y>0 entries with Y<H
+    const int32_t* zrow;           // [N+1] CSR row offsets of the Y = 0 entries
+    const uint16_t* zcol;          // [nz]  their genes
+    const int32_t* big_off;        // [B+1] static counts Y >= H, per batch
+    const int32_t* big;            // [nbig] their values
 
     // state
     int* iter;                     // [1] global iteration (sweep) index
@@ -54,10 +59,7 @@ struct ChainDev {
     uint32_t* hist_d;              // [B*H] dynamic histogram (entries with Y = 0)
     uint64_t* gstat;               // [B*5] n_z1, sum x z, over0, over1_rej, over1_acc
 
-    // L, p, tau0
-    int32_t* lsum;                 // [G] sum_k L_kg
-    uint64_t* bsq;                 // [G] exact sum over L=0 of beta^2
-    int32_t* beta_acc;             // [G] bitmask of accepted beta_kg
+    int32_t* beta_acc;             // [K*G] 1 where this sweep accepted beta_kg
 
     // posterior accumulators
     int* n_rec;
@@ -75,6 +77,9 @@ struct HostInit {
     std::vector<double> alpha, beta, nu, delta, phi, gamma, pi, ptau;
     std::vector<double> mu_alpha, mu_nu, mu_delta;
     std::vector<int64_t> hist_s;
+    std::vector<int32_t> zrow;     // CSR of the Y = 0 entries
+    std::vector<uint16_t> zcol;
+    std::vector<int32_t> big_off, big;   // static counts >= the histogram size, per batch
 };
 
 // host.cpp

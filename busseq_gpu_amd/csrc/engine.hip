@@ -1,23 +1,26 @@
 // engine.hip -- gfx950 kernels of one BUSseq Gibbs sweep and the host driver
 // that allocates the chains, captures the sweep into a hipGraph and replays it.
 //
-// Sweep (one hipGraph, thirteen launches, chains on blockIdx.z):
+// Sweep (one hipGraph, fifteen launches, chains on blockIdx.z):
 //   k_prep       alpha/gamma proposals, zero the per-sweep counters
 //   k_w          W_i: categorical full conditional, one wave per cell     [row pass]
 //   k_plan       pi_b Dirichlet draw, stable sort of cells by (b, W), tiles
-//   k_xz_alpha   X/Z imputation, gamma histogram, alpha partial sums     [tile pass]
+//   k_xz         X/Z imputation of the Y = 0 entries, gamma histogram     [row pass]
+//   k_alpha      alpha partial sums                                      [tile pass]
 //   k_alpha_dec  alpha accept, L draw, beta/nu/phi proposals
 //   k_gamma_ptau gamma0/gamma1 accept per batch; p, tau0 draw
 //   k_beta_nu    beta and nu partial sums (nu under both beta outcomes)  [tile pass]
-//   k_bn_dec     beta then nu accept
+//   k_b_dec      beta accept, per (gene, subtype)
+//   k_n_dec      nu accept with the accepted beta branch, per (gene, batch)
 //   k_delta      delta proposal, row sum and accept, one wave per cell   [row pass]
 //   k_phi        phi partial sums with the new delta                     [tile pass]
 //   k_phi_dec    phi accept
 //   k_record     posterior accumulators and traces after burn-in
 //   k_tick       iteration counter
 //
-// Row passes: lane l of the wave owning cell i sums genes g = l (mod 64) in
-// double precision; the 64 lane sums are added exactly in fixed point.
+// Row passes: genes are cut into chunks of BQ_CHUNK = 1024; lane l of the wave
+// owning cell i sums the genes g = l (mod 64) of each chunk in double
+// precision and the (chunk, lane) sums are added exactly in fixed point.
 // Tile passes: a workgroup owns one tile (<= TR cells of one (batch, subtype)
 // group, in stable sorted order) and 256 genes; each lane sums its gene over
 // the tile in double precision and stores the exact fixed-point value.  The
@@ -127,9 +130,10 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
 
 // ---------------------------------------------------------------------------
 // k_w: subtype draw from the categorical full conditional (replaces the MH
-// step of cu:820-854); logit_k = log pi_bk + sum_g [x beta_k - (phi+x) log1p(r0 expm1 beta_k)]
+// step of cu:820-854); logit_k = log pi_bk + sum_g [x beta_k - (phi+x) log1p(r0 expm1 beta_k)].
+// One wave per cell; KT = K at compile time.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int KT>
 __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
 {
     const ChainDev& c = cds[blockIdx.z];
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= c.N) return;
     const int t = *c.iter;
-    const int G = c.G, K = c.K, b = c.batch[i];
+    const int G = c.G, b = c.batch[i];
     const T* ea = (const T*)c.ea;
     const T* en = (const T*)c.en + (size_t)b * G;
     const T* ph = (const T*)c.ph + (size_t)b * G;
@@ -146,44 +150,61 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
     const T edi = ((const T*)c.ed)[i];
     const int32_t* row = c.S + (size_t)i * c.Gp;
 
-    double part[BQ_KMAX];
+    uint64_t acc[KT];
 #pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k) part[k] = 0.0;
-    for (int g = lane; g < G; g += 64) {
-        int32_t s = row[g];
-        int xi = s < 0 ? ~s : s;
-        T x = (T)xi;
-        T phi = ph[g];
-        T mu0 = tmu(ea[g], (T)1, en[g], edi);
-        T r0 = tratio(mu0, phi);
+    for (int k = 0; k < KT; ++k) acc[k] = 0;
+    for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
+        const int c1 = c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G;
+        double part[KT];
 #pragma unroll
-        for (int k = 1; k < BQ_KMAX; ++k)
-            if (k < K) part[k] += (double)ttw(x, phi, r0, bt[(size_t)k * G + g], em[(size_t)k * G + g]);
+        for (int k = 0; k < KT; ++k) part[k] = 0.0;
+        for (int g = c0 + lane; g < c1; g += 64) {
+            int32_t s = row[g];
+            int xi = s < 0 ? ~s : s;
+            T x = (T)xi;
+            T phi = ph[g];
+            T mu0 = tmu(ea[g], (T)1, en[g], edi);
+            T r0 = tratio(mu0, phi);
+#pragma unroll
+            for (int k = 1; k < KT; ++k)
+                part[k] += (double)ttw(x, phi, r0, bt[(size_t)k * G + g], em[(size_t)k * G + g]);
+        }
+#pragma unroll
+        for (int k = 1; k < KT; ++k) acc[k] += bq_fix(part[k]);
     }
-    double lp[BQ_KMAX];
+    double lp[KT];
     double mx = -BQ_INF;
 #pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k) {
-        if (k < K) {
-            uint64_t a = 0;
-            if (k > 0) a = wave_sum_u64(bq_fix(part[k]));
-            double lk = bq_unfix(a);
-            lp[k] = lk + bq_log(c.pi[b * K + k]);
-            if (lp[k] > mx) mx = lp[k];
-        }
+    for (int k = 0; k < KT; ++k) {
+        uint64_t a = k > 0 ? wave_sum_u64(acc[k]) : 0;
+        double lk = bq_unfix(a);
+        lp[k] = lk + bq_log(c.pi[b * KT + k]);
+        if (lp[k] > mx) mx = lp[k];
     }
     double tot = 0.0;
 #pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k)
-        if (k < K) { lp[k] = bq_exp(lp[k] - mx); tot += lp[k]; }
+    for (int k = 0; k < KT; ++k) { lp[k] = bq_exp(lp[k] - mx); tot += lp[k]; }
     double tgt = ph_uniform(c, (uint32_t)i, 0u, t, BQ_PH_W) * tot, cum = 0.0;
-    int kk = K - 1, found = 0;
+    int kk = KT - 1, found = 0;
 #pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k)
-        if (k < K && !found) { cum += lp[k]; if (tgt <= cum) { kk = k; found = 1; } }
+    for (int k = 0; k < KT; ++k)
+        if (!found) { cum += lp[k]; if (tgt <= cum) { kk = k; found = 1; } }
     if (lane == 0) {
         c.W[i] = kk;
-        atomicAdd(&c.cnt[b * K + kk], 1);
+        atomicAdd(&c.cnt[b * KT + kk], 1);
+    }
+}
+
+template <typename T>
+void launch_w(int K, dim3 grid, dim3 blk, hipStream_t st, const ChainDev* cd)
+{
+    switch (K) {
+#define BQ_W_CASE(KV) case KV: hipLaunchKernelGGL((k_w<T, KV>), grid, blk, 0, st, cd); break;
+        BQ_W_CASE(2) BQ_W_CASE(3) BQ_W_CASE(4) BQ_W_CASE(5) BQ_W_CASE(6) BQ_W_CASE(7) BQ_W_CASE(8)
+        BQ_W_CASE(9) BQ_W_CASE(10) BQ_W_CASE(11) BQ_W_CASE(12) BQ_W_CASE(13) BQ_W_CASE(14)
+        BQ_W_CASE(15) BQ_W_CASE(16)
+#undef BQ_W_CASE
+    default: break;
     }
 }
 
@@ -269,166 +290,198 @@ __global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
 }
 
 // ---------------------------------------------------------------------------
-// k_xz_alpha: X/Z imputation (cu:484-512), dynamic gamma histogram, alpha
-// partial sums (cu:608-623).  Block = 256 genes x one tile.
+// k_xz: X/Z imputation (cu:484-512) of the Y = 0 entries, one wave per cell,
+// lanes over the cell's static list of zero entries; dynamic gamma histogram.
+// An entry is written back only when its state changes.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_xz_alpha(const ChainDev* __restrict__ cds)
+__global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
 {
     const ChainDev& c = cds[blockIdx.z];
-    const int tile = blockIdx.y;
-    if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
-    __shared__ uint32_t hist[H];
-    __shared__ uint64_t s_red[4];
-    const int tid = threadIdx.x;
-    for (int x = tid; x < H; x += 256) hist[x] = 0u;
-    __syncthreads();
+    __shared__ uint32_t hist[4][H];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wv;
+    if (i >= c.N) return;
+    uint32_t* hw = hist[wv];
+    for (int x = lane; x < H; x += 64) hw[x] = 0u;
     const int G = c.G, B = c.B, t = *c.iter;
-    const int b = c.tiles[4 * tile], k = c.tiles[4 * tile + 1];
-    const int st = c.tiles[4 * tile + 2], nr = c.tiles[4 * tile + 3];
-    const int g = blockIdx.x * 256 + tid;
+    const int b = c.batch[i], k = c.W[i];
     const double g0 = c.gamma[b], g1 = c.gamma[B + b];
     const double g0p = c.gamma_p[b], g1p = c.gamma_p[B + b];
     const double qz = 1.0 / (1.0 + bq_exp(g0));
-    uint64_t my_n1 = 0, my_sx1 = 0;
-    if (g < G) {
-        const size_t bg = (size_t)b * G + g;
-        const T eag = ((const T*)c.ea)[g], ebg = ((const T*)c.eb)[(size_t)k * G + g];
-        const T eng = ((const T*)c.en)[bg], phg = ((const T*)c.ph)[bg];
-        const T dag = ((const T*)c.da)[g], cag = ((const T*)c.ca)[g];
-        const T* ed = (const T*)c.ed;
-        const double phid = c.phi[bg];
-        double acc = 0.0;
-        for (int rr = 0; rr < nr; ++rr) {
-            const int i = c.perm[st + rr];
-            const size_t e = (size_t)i * c.Gp + g;
-            const int32_t s = c.S[e];
-            const T mu = tmu(eag, ebg, eng, ed[i]);
-            int xi;
-            if (s <= 0) {
-                bq_rng r;
-                bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ);
-                int xo = s < 0 ? ~s : 0;
-                int z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
-                if (z) {
-                    int xp = bq_rnbinom(&r, (double)mu, phid);
-                    double lr = bq_neg_logit_sp(g0, g1, (double)xo) - bq_neg_logit_sp(g0, g1, (double)xp);
-                    xi = bq_log(bq_uniform(&r)) <= lr ? xp : xo;
-                    c.S[e] = ~xi;
-                    my_n1 += 1;
-                    my_sx1 += (uint64_t)xi;
-                } else {
-                    xi = 0;
-                    c.S[e] = 0;
-                }
-                if (xi < H) atomicAdd(&hist[xi], 1u);
-            } else {
-                xi = s;
-            }
-            if (xi >= H) {
-                double xd = (double)xi;
-                double a0 = bq_sp_lin(g0, g1, xd), a1 = bq_sp_lin(g0p, g1, xd);
-                double a2 = bq_sp_lin(g0, g1p, xd), a3 = bq_sp_lin(g0p, g1p, xd);
-                atomicAdd((unsigned long long*)&c.gstat[b * 5 + 2], (unsigned long long)bq_fix(a0 - a1));
-                atomicAdd((unsigned long long*)&c.gstat[b * 5 + 3], (unsigned long long)bq_fix(a0 - a2));
-                atomicAdd((unsigned long long*)&c.gstat[b * 5 + 4], (unsigned long long)bq_fix(a1 - a3));
-            }
-            const T rt = tratio(mu, phg);
-            acc += (double)ttmh((T)xi, phg, rt, dag, cag);
+    const T edi = ((const T*)c.ed)[i];
+    const T* ea = (const T*)c.ea;
+    const T* eb = (const T*)c.eb + (size_t)k * G;
+    const T* en = (const T*)c.en + (size_t)b * G;
+    const double* phi = c.phi + (size_t)b * G;
+    int32_t* row = c.S + (size_t)i * c.Gp;
+    const int z0 = c.zrow[i], z1 = c.zrow[i + 1];
+    uint64_t n1 = 0, sx1 = 0, o0 = 0, o1r = 0, o1a = 0;
+    for (int p = z0 + lane; p < z1; p += 64) {
+        const int g = c.zcol[p];
+        const int32_t s = row[g];
+        const T mu = tmu(ea[g], eb[g], en[g], edi);
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ);
+        int xo = s < 0 ? ~s : 0;
+        int z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
+        int32_t sn = 0;
+        if (z) {
+            int xp = bq_rnbinom(&r, (double)mu, phi[g]);
+            double lr = bq_neg_logit_sp(g0, g1, (double)xo) - bq_neg_logit_sp(g0, g1, (double)xp);
+            int xi = bq_log(bq_uniform(&r)) <= lr ? xp : xo;
+            sn = ~xi;
         }
-        c.part[((size_t)tile * 3 + 0) * c.Gp + g] = bq_fix(acc);
+        if (sn != s) row[g] = sn;
+        const int xi = sn < 0 ? ~sn : 0;
+        if (sn < 0) { n1 += 1; sx1 += (uint64_t)xi; }
+        if (xi < H) {
+            atomicAdd(&hw[xi], 1u);
+        } else {
+            double xd = (double)xi;
+            double a0 = bq_sp_lin(g0, g1, xd), a1 = bq_sp_lin(g0p, g1, xd);
+            double a2 = bq_sp_lin(g0, g1p, xd), a3 = bq_sp_lin(g0p, g1p, xd);
+            o0 += bq_fix(a0 - a1);
+            o1r += bq_fix(a0 - a2);
+            o1a += bq_fix(a1 - a3);
+        }
     }
-    uint64_t n1 = block_sum_u64(my_n1, s_red);
-    uint64_t sx1 = block_sum_u64(my_sx1, s_red);
-    for (int x = tid; x < H; x += 256) {
-        uint32_t v = hist[x];
+    n1 = wave_sum_u64(n1);
+    sx1 = wave_sum_u64(sx1);
+    o0 = wave_sum_u64(o0);
+    o1r = wave_sum_u64(o1r);
+    o1a = wave_sum_u64(o1a);
+    for (int x = lane; x < H; x += 64) {
+        uint32_t v = hw[x];
         if (v) atomicAdd(&c.hist_d[b * H + x], v);
     }
-    if (tid == 0) {
-        if (n1) atomicAdd((unsigned long long*)&c.gstat[b * 5 + 0], (unsigned long long)n1);
-        if (sx1) atomicAdd((unsigned long long*)&c.gstat[b * 5 + 1], (unsigned long long)sx1);
+    if (lane == 0) {
+        unsigned long long* gs = (unsigned long long*)(c.gstat + b * 5);
+        if (n1) atomicAdd(gs + 0, (unsigned long long)n1);
+        if (sx1) atomicAdd(gs + 1, (unsigned long long)sx1);
+        if (o0 || o1r || o1a) {
+            atomicAdd(gs + 2, (unsigned long long)o0);
+            atomicAdd(gs + 3, (unsigned long long)o1r);
+            atomicAdd(gs + 4, (unsigned long long)o1a);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_alpha_dec: alpha accept (cu:625-632); L Gibbs (cu:634-648) and its sums for
-// p/tau0; beta (cu:670-674), nu (cu:709-713) and phi (cu:780-787) proposals.
+// k_alpha: alpha partial sums (cu:608-623).  Block = 256 genes x one tile.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_alpha(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int tile = blockIdx.y;
+    if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= c.G) return;
+    const int G = c.G;
+    const int b = c.tiles[4 * tile], k = c.tiles[4 * tile + 1];
+    const int st = c.tiles[4 * tile + 2], nr = c.tiles[4 * tile + 3];
+    const size_t bg = (size_t)b * G + g;
+    const T eag = ((const T*)c.ea)[g], ebg = ((const T*)c.eb)[(size_t)k * G + g];
+    const T eng = ((const T*)c.en)[bg], phg = ((const T*)c.ph)[bg];
+    const T dag = ((const T*)c.da)[g], cag = ((const T*)c.ca)[g];
+    const T* ed = (const T*)c.ed;
+    double acc = 0.0;
+    for (int rr = 0; rr < nr; ++rr) {
+        const int i = c.perm[st + rr];
+        const int32_t s = c.S[(size_t)i * c.Gp + g];
+        const int xi = s < 0 ? ~s : s;
+        const T mu = tmu(eag, ebg, eng, ed[i]);
+        const T rt = tratio(mu, phg);
+        acc += (double)ttmh((T)xi, phg, rt, dag, cag);
+    }
+    c.part[((size_t)tile * 3) * c.Gp + g] = bq_fix(acc);
+}
+
+// ---------------------------------------------------------------------------
+// k_alpha_dec: flat work items
+//   [0, G)                 alpha accept (cu:625-632)
+//   + G(K-1)  (g, k>=1)    L Gibbs draw (cu:634-648), beta proposal (cu:670-674)
+//   + G(B-1)  (g, b>=1)    nu proposal (cu:709-713)
+//   + G B     (g, b)       phi proposal (cu:780-787) and the phi-term constants
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ cds)
 {
     const ChainDev& c = cds[blockIdx.z];
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= c.G) return;
-    const int G = c.G, K = c.K, B = c.B, t = *c.iter, nt = *c.ntiles;
-    const size_t Gp = c.Gp;
-    uint64_t a = 0;
-    for (int tt = 0; tt < nt; ++tt) a += c.part[((size_t)tt * 3) * Gp + g];
-    double sum = bq_unfix(a);
-    double al = c.alpha[g], ap = c.alpha_p[g];
-    double lr = sum + bq_lr_normal(al, ap, c.mu_alpha[g], BQ_SIGMA2_ALPHA);
-    if (bq_log(ph_uniform(c, (uint32_t)g, 0u, t, BQ_PH_ALPHA_A)) <= lr) {
-        c.alpha[g] = ap;
-        ((T*)c.ea)[g] = (T)bq_exp(ap);
-    }
-    // L with the current p, tau0
-    const double p = c.ptau[0], tau0 = c.ptau[1];
-    int ls = 0;
-    uint64_t bs = 0;
-    for (int k = 1; k < K; ++k) {
-        size_t j = (size_t)k * G + g;
-        double be = c.beta[j];
-        int L = bq_bern_logodds(ph_uniform(c, (uint32_t)g, (uint32_t)k, t, BQ_PH_L), bq_logodds_L(p, tau0, be));
-        c.L[j] = L;
-        if (L) ls += 1; else bs += bq_fix(be * be);
-    }
-    c.lsum[g] = ls;
-    c.bsq[g] = bs;
-    // beta, nu proposals and their T constants
-    for (int k = 0; k < K; ++k) {
-        size_t j = (size_t)k * G + g;
-        double be = c.beta[j], bp = be;
-        if (k > 0) {
-            bq_rng r;
-            bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)k, (uint32_t)t, BQ_PH_BETA_P);
-            bp = be + BQ_RW_SD * bq_normal(&r);
+    const int G = c.G, K = c.K, B = c.B;
+    int j = blockIdx.x * 256 + threadIdx.x;
+    const int t = *c.iter;
+    if (j < G) {
+        const int g = j, nt = *c.ntiles;
+        const size_t stride = (size_t)3 * c.Gp;
+        const uint64_t* pp = c.part + g;
+        uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        int tt = 0;
+        for (; tt + 4 <= nt; tt += 4) {
+            a0 += pp[(size_t)tt * stride];
+            a1 += pp[(size_t)(tt + 1) * stride];
+            a2 += pp[(size_t)(tt + 2) * stride];
+            a3 += pp[(size_t)(tt + 3) * stride];
         }
-        c.beta_p[j] = bp;
+        for (; tt < nt; ++tt) a0 += pp[(size_t)tt * stride];
+        double sum = bq_unfix((a0 + a1) + (a2 + a3));
+        double al = c.alpha[g], ap = c.alpha_p[g];
+        double lr = sum + bq_lr_normal(al, ap, c.mu_alpha[g], BQ_SIGMA2_ALPHA);
+        if (bq_log(ph_uniform(c, (uint32_t)g, 0u, t, BQ_PH_ALPHA_A)) <= lr) {
+            c.alpha[g] = ap;
+            ((T*)c.ea)[g] = (T)bq_exp(ap);
+        }
+        return;
+    }
+    j -= G;
+    if (j < G * (K - 1)) {
+        const int k = 1 + j / G, g = j % G;
+        const size_t q = (size_t)k * G + g;
+        const double be = c.beta[q];
+        c.L[q] = bq_bern_logodds(ph_uniform(c, (uint32_t)g, (uint32_t)k, t, BQ_PH_L),
+                                 bq_logodds_L(c.ptau[0], c.ptau[1], be));
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)k, (uint32_t)t, BQ_PH_BETA_P);
+        double bp = be + BQ_RW_SD * bq_normal(&r);
+        c.beta_p[q] = bp;
         double dl = bp - be;
-        ((T*)c.db)[j] = (T)dl;
-        ((T*)c.cb)[j] = (T)bq_expm1(dl);
+        ((T*)c.db)[q] = (T)dl;
+        ((T*)c.cb)[q] = (T)bq_expm1(dl);
+        return;
     }
-    for (int b = 0; b < B; ++b) {
-        size_t j = (size_t)b * G + g;
-        double nv = c.nu[j], np = nv;
-        if (b > 0) {
-            bq_rng r;
-            bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)b, (uint32_t)t, BQ_PH_NU_P);
-            np = nv + BQ_RW_SD * bq_normal(&r);
-        }
-        c.nu_p[j] = np;
+    j -= G * (K - 1);
+    if (j < G * (B - 1)) {
+        const int b = 1 + j / G, g = j % G;
+        const size_t q = (size_t)b * G + g;
+        const double nv = c.nu[q];
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)b, (uint32_t)t, BQ_PH_NU_P);
+        double np = nv + BQ_RW_SD * bq_normal(&r);
+        c.nu_p[q] = np;
         double dl = np - nv;
-        ((T*)c.dn)[j] = (T)dl;
-        ((T*)c.cn)[j] = (T)bq_expm1(dl);
+        ((T*)c.dn)[q] = (T)dl;
+        ((T*)c.cn)[q] = (T)bq_expm1(dl);
+        return;
     }
-    // phi proposals and the per-(b,g) constants of the phi terms
-    const size_t BG = (size_t)B * G;
-    T* pc = (T*)c.pc;
-    for (int b = 0; b < B; ++b) {
-        size_t j = (size_t)b * G + g;
+    j -= G * (B - 1);
+    if (j < G * B) {
+        const int b = j / G, g = j % G;
+        const size_t q = (size_t)b * G + g, BG = (size_t)B * G;
+        T* pc = (T*)c.pc;
         bq_rng r;
         bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)b, (uint32_t)t, BQ_PH_PHI_P);
-        double f = c.phi[j];
+        double f = c.phi[q];
         double fp = bq_rgamma(&r, f, 1.0);
-        c.phi_p[j] = fp;
+        c.phi_p[q] = fp;
         if (!(fp > 0.0)) fp = f;
         double d = fp - f;
-        pc[j] = (T)fp;
-        pc[BG + j] = (T)d;
-        pc[2 * BG + j] = (T)(d / f);
-        pc[3 * BG + j] = (T)(1.0 / fp);
-        pc[4 * BG + j] = (T)bq_log1p(d / f);
-        pc[5 * BG + j] = (T)bq_lgamma_diff64(f, d);
+        pc[q] = (T)fp;
+        pc[BG + q] = (T)d;
+        pc[2 * BG + q] = (T)(d / f);
+        pc[3 * BG + q] = (T)(1.0 / fp);
+        pc[4 * BG + q] = (T)bq_log1p(d / f);
+        pc[5 * BG + q] = (T)bq_lgamma_diff64(f, d);
     }
 }
 
@@ -447,7 +500,10 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
         if (t < c.n_unchanged) return;
         const int G = c.G, K = c.K;
         uint64_t cl = 0, sb = 0;
-        for (int g = tid; g < G; g += 256) { cl += (uint64_t)c.lsum[g]; sb += c.bsq[g]; }
+        for (int q = G + tid; q < K * G; q += 256) {
+            if (c.L[q]) cl += 1;
+            else { double be = c.beta[q]; sb += bq_fix(be * be); }
+        }
         cl = block_sum_u64(cl, s_red);
         sb = block_sum_u64(sb, s_red);
         if (tid == 0) {
@@ -473,6 +529,11 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
         int64_t n = hs[x] + (int64_t)hd[x];
         if (n) a0 += bq_fix((double)n * (bq_sp_lin(g0, g1, (double)x) - bq_sp_lin(g0p, g1, (double)x)));
     }
+    const int bb0 = c.big_off[b], bb1 = c.big_off[b + 1];
+    for (int q = bb0 + tid; q < bb1; q += 256) {
+        double xd = (double)c.big[q];
+        a0 += bq_fix(bq_sp_lin(g0, g1, xd) - bq_sp_lin(g0p, g1, xd));
+    }
     a0 = block_sum_u64(a0, s_red);
     if (tid == 0) {
         const uint64_t* gs = c.gstat + b * 5;
@@ -491,6 +552,10 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
     for (int x = tid; x < H; x += 256) {
         int64_t n = hs[x] + (int64_t)hd[x];
         if (n) a1 += bq_fix((double)n * (bq_sp_lin(g0n, g1, (double)x) - bq_sp_lin(g0n, g1p, (double)x)));
+    }
+    for (int q = bb0 + tid; q < bb1; q += 256) {
+        double xd = (double)c.big[q];
+        a1 += bq_fix(bq_sp_lin(g0n, g1, xd) - bq_sp_lin(g0n, g1p, xd));
     }
     a1 = block_sum_u64(a1, s_red);
     if (tid == 0) {
@@ -553,63 +618,64 @@ __global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cd
 }
 
 // ---------------------------------------------------------------------------
-// k_bn_dec: beta accept (cu:696-707) for all k, then nu accept (cu:730-737)
-// with the nu sums of the accepted beta branch
+// k_b_dec: beta accept (cu:696-707), one item per (g, k >= 1)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_bn_dec(const ChainDev* __restrict__ cds)
+__global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
 {
     const ChainDev& c = cds[blockIdx.z];
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= c.G) return;
-    const int G = c.G, K = c.K, B = c.B, t = *c.iter;
+    const int G = c.G, K = c.K, B = c.B;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= G * (K - 1)) return;
+    const int k = 1 + j / G, g = j % G, t = *c.iter;
     const size_t Gp = c.Gp;
     const int* toff = c.tile_off;
-    uint64_t ab[BQ_KMAX];
-#pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k) ab[k] = 0;
+    uint64_t a = 0;
     for (int b = 0; b < B; ++b) {
-#pragma unroll
-        for (int k = 1; k < BQ_KMAX; ++k) {
-            if (k < K) {
-                int e = toff[b * K + k + 1];
-                for (int tt = toff[b * K + k]; tt < e; ++tt) ab[k] += c.part[((size_t)tt * 3) * Gp + g];
-            }
-        }
+        int e = toff[b * K + k + 1];
+        for (int tt = toff[b * K + k]; tt < e; ++tt) a += c.part[((size_t)tt * 3) * Gp + g];
     }
-    const double tau0 = c.ptau[1];
-    uint32_t mask = 0;
-#pragma unroll
-    for (int k = 1; k < BQ_KMAX; ++k) {
-        if (k < K) {
-            size_t j = (size_t)k * G + g;
-            double sum = bq_unfix(ab[k]);
-            double be = c.beta[j], bp = c.beta_p[j];
-            double lr = sum + bq_lr_beta(be, bp, c.L[j], tau0);
-            if (bq_log(ph_uniform(c, (uint32_t)g, (uint32_t)k, t, BQ_PH_BETA_A)) <= lr) {
-                c.beta[j] = bp;
-                ((T*)c.eb)[j] = (T)bq_exp(bp);
-                ((T*)c.em)[j] = (T)bq_expm1(bp);
-                ((T*)c.bt)[j] = (T)bp;
-                mask |= 1u << k;
-            }
-        }
+    const size_t q = (size_t)k * G + g;
+    double sum = bq_unfix(a);
+    double be = c.beta[q], bp = c.beta_p[q];
+    double lr = sum + bq_lr_beta(be, bp, c.L[q], c.ptau[1]);
+    int acc = bq_log(ph_uniform(c, (uint32_t)g, (uint32_t)k, t, BQ_PH_BETA_A)) <= lr;
+    c.beta_acc[q] = acc;
+    if (acc) {
+        c.beta[q] = bp;
+        ((T*)c.eb)[q] = (T)bq_exp(bp);
+        ((T*)c.em)[q] = (T)bq_expm1(bp);
+        ((T*)c.bt)[q] = (T)bp;
     }
-    for (int b = 1; b < B; ++b) {
-        uint64_t a = 0;
-        for (int k = 0; k < K; ++k) {
-            int slot = (k >= 1 && ((mask >> k) & 1u)) ? 2 : 1;
-            int e = toff[b * K + k + 1];
-            for (int tt = toff[b * K + k]; tt < e; ++tt) a += c.part[((size_t)tt * 3 + slot) * Gp + g];
-        }
-        size_t j = (size_t)b * G + g;
-        double sum = bq_unfix(a);
-        double nv = c.nu[j], np = c.nu_p[j];
-        double lr = sum + bq_lr_normal(nv, np, c.mu_nu[b], BQ_SIGMA2_NU);
-        if (bq_log(ph_uniform(c, (uint32_t)g, (uint32_t)b, t, BQ_PH_NU_A)) <= lr) {
-            c.nu[j] = np;
-            ((T*)c.en)[j] = (T)bq_exp(np);
-        }
+}
+
+// ---------------------------------------------------------------------------
+// k_n_dec: nu accept (cu:730-737), one item per (g, b >= 1), using the nu sums
+// of the accepted beta branch of each subtype
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_n_dec(const ChainDev* __restrict__ cds)
+{
+    const ChainDev& c = cds[blockIdx.z];
+    const int G = c.G, K = c.K, B = c.B;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= G * (B - 1)) return;
+    const int b = 1 + j / G, g = j % G, t = *c.iter;
+    const size_t Gp = c.Gp;
+    const int* toff = c.tile_off;
+    uint64_t a = 0;
+    for (int k = 0; k < K; ++k) {
+        const int slot = (k >= 1 && c.beta_acc[(size_t)k * G + g]) ? 2 : 1;
+        const int e = toff[b * K + k + 1];
+        for (int tt = toff[b * K + k]; tt < e; ++tt) a += c.part[((size_t)tt * 3 + slot) * Gp + g];
+    }
+    const size_t q = (size_t)b * G + g;
+    double sum = bq_unfix(a);
+    double nv = c.nu[q], np = c.nu_p[q];
+    double lr = sum + bq_lr_normal(nv, np, c.mu_nu[b], BQ_SIGMA2_NU);
+    if (bq_log(ph_uniform(c, (uint32_t)g, (uint32_t)b, t, BQ_PH_NU_A)) <= lr) {
+        c.nu[q] = np;
+        ((T*)c.en)[q] = (T)bq_exp(np);
     }
 }
 
@@ -641,16 +707,21 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
     const double dl = dp - del;
     const T dd = (T)dl, cd = (T)bq_expm1(dl);
     const int32_t* row = c.S + (size_t)i * c.Gp;
-    double part = 0.0;
-    for (int g = lane; g < G; g += 64) {
-        int32_t s = row[g];
-        int xi = s < 0 ? ~s : s;
-        T phi = ph[g];
-        T mu = tmu(ea[g], eb[g], en[g], edi);
-        T rt = tratio(mu, phi);
-        part += (double)ttmh((T)xi, phi, rt, dd, cd);
+    uint64_t a = 0;
+    for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
+        const int c1 = c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G;
+        double part = 0.0;
+        for (int g = c0 + lane; g < c1; g += 64) {
+            int32_t s = row[g];
+            int xi = s < 0 ? ~s : s;
+            T phi = ph[g];
+            T mu = tmu(ea[g], eb[g], en[g], edi);
+            T rt = tratio(mu, phi);
+            part += (double)ttmh((T)xi, phi, rt, dd, cd);
+        }
+        a += bq_fix(part);
     }
-    uint64_t a = wave_sum_u64(bq_fix(part));
+    a = wave_sum_u64(a);
     double sum = bq_unfix(a);
     double lr = sum + bq_lr_normal(del, dp, c.mu_delta[i], BQ_SIGMA2_DELTA);
     if (bq_log(ph_uniform(c, (uint32_t)i, 0u, t, BQ_PH_DELTA_A)) <= lr) {
@@ -694,30 +765,29 @@ __global__ __launch_bounds__(256) void k_phi(const ChainDev* __restrict__ cds)
 }
 
 // ---------------------------------------------------------------------------
-// k_phi_dec: phi accept (cu:806-818)
+// k_phi_dec: phi accept (cu:806-818), one item per (g, b)
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_phi_dec(const ChainDev* __restrict__ cds)
 {
     const ChainDev& c = cds[blockIdx.z];
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= c.G) return;
-    const int G = c.G, K = c.K, B = c.B, t = *c.iter;
+    const int G = c.G, K = c.K, B = c.B;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= G * B) return;
+    const int b = j / G, g = j % G, t = *c.iter;
     const size_t Gp = c.Gp;
-    for (int b = 0; b < B; ++b) {
-        uint64_t a = 0;
-        int e = c.tile_off[(b + 1) * K];
-        for (int tt = c.tile_off[b * K]; tt < e; ++tt) a += c.part[((size_t)tt * 3) * Gp + g];
-        size_t j = (size_t)b * G + g;
-        double sum = bq_unfix(a);
-        double f = c.phi[j], fp = c.phi_p[j];
-        double u = ph_uniform(c, (uint32_t)g, (uint32_t)b, t, BQ_PH_PHI_A);
-        if (!(fp > 0.0)) continue;
-        double lr = sum + bq_lr_phi(f, fp);
-        if (bq_log(u) <= lr) {
-            c.phi[j] = fp;
-            ((T*)c.ph)[j] = (T)fp;
-        }
+    uint64_t a = 0;
+    const int e = c.tile_off[(b + 1) * K];
+    for (int tt = c.tile_off[b * K]; tt < e; ++tt) a += c.part[((size_t)tt * 3) * Gp + g];
+    const size_t q = (size_t)b * G + g;
+    double sum = bq_unfix(a);
+    double f = c.phi[q], fp = c.phi_p[q];
+    double u = ph_uniform(c, (uint32_t)g, (uint32_t)b, t, BQ_PH_PHI_A);
+    if (!(fp > 0.0)) return;
+    double lr = sum + bq_lr_phi(f, fp);
+    if (bq_log(u) <= lr) {
+        c.phi[q] = fp;
+        ((T*)c.ph)[q] = (T)fp;
     }
 }
 
@@ -801,14 +871,19 @@ __global__ __launch_bounds__(256) void k_bic(const ChainDev* __restrict__ cds, c
     const double* phi = de + N;
     const double* ga = phi + (size_t)B * G;
     const int32_t* Srow = c.S + (size_t)i * c.Gp;
-    double part = 0.0;
-    for (int g = lane; g < G; g += 64) {
-        int32_t s = Srow[g];
-        int y = s > 0 ? s : 0;
-        part += bq_bic_elem(al[g], be[(size_t)k * G + g], nu[(size_t)b * G + g], de[i], ga[b], ga[B + b],
-                            phi[(size_t)b * G + g], y);
+    uint64_t a = 0;
+    for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
+        const int c1 = c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G;
+        double part = 0.0;
+        for (int g = c0 + lane; g < c1; g += 64) {
+            int32_t s = Srow[g];
+            int y = s > 0 ? s : 0;
+            part += bq_bic_elem(al[g], be[(size_t)k * G + g], nu[(size_t)b * G + g], de[i], ga[b], ga[B + b],
+                                phi[(size_t)b * G + g], y);
+        }
+        a += bq_fix(part);
     }
-    uint64_t a = wave_sum_u64(bq_fix(part));
+    a = wave_sum_u64(a);
     if (lane == 0) ll[(size_t)i * K + k] = bq_unfix(a);
 }
 
@@ -853,9 +928,9 @@ struct busseq_sampler {
 namespace {
 
 static const char* const kKernelNames[] = {
-    "k_prep", "k_w", "k_plan", "k_xz_alpha", "k_alpha_dec", "k_gamma_ptau", "k_beta_nu",
-    "k_bn_dec", "k_delta", "k_phi", "k_phi_dec", "k_record", "k_tick"};
-static const int kNumKernels = 13;
+    "k_prep", "k_w", "k_plan", "k_xz", "k_alpha", "k_alpha_dec", "k_gamma_ptau", "k_beta_nu",
+    "k_b_dec", "k_n_dec", "k_delta", "k_phi", "k_phi_dec", "k_record", "k_tick"};
+static const int kNumKernels = 15;
 
 // ev: optional array of kNumKernels + 1 events recorded around each launch
 template <typename T>
@@ -871,7 +946,11 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     const dim3 g_rows((N + 3) / 4, 1, C);
     const dim3 g_plan(B, 1, C);
     const dim3 g_tile((G + 255) / 256, s->T_max, C);
-    const dim3 g_gene((G + 255) / 256, 1, C);
+    const int n_ad = G * (1 + (K - 1) + (B - 1) + B);
+    const dim3 g_ad((n_ad + 255) / 256, 1, C);
+    const dim3 g_gk((G * (K - 1) + 255) / 256, 1, C);
+    const dim3 g_gb1((G * (B - 1) + 255) / 256, 1, C);
+    const dim3 g_gb((G * B + 255) / 256, 1, C);
     const dim3 g_gamma(B + 1, 1, C);
     int rec = N;
     if (K * G > rec) rec = K * G;
@@ -884,16 +963,18 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     int q = 0;
 #define EV() do { if (ev) HIPCHK(hipEventRecord(ev[q], st)); ++q; } while (0)
     EV(); hipLaunchKernelGGL(k_prep<T>, g_prep, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_w<T>, g_rows, blk, 0, st, cd);
+    EV(); launch_w<T>(K, g_rows, blk, st, cd);
     EV(); hipLaunchKernelGGL(k_plan, g_plan, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_xz_alpha<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_gene, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_xz<T>, g_rows, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_alpha<T>, g_tile, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_ad, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_beta_nu<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_bn_dec<T>, g_gene, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_b_dec<T>, g_gk, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_n_dec<T>, g_gb1, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_phi_dec<T>, g_gene, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_phi_dec<T>, g_gb, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_record, g_rec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_tick, dim3(1, 1, C), dim3(64), 0, st, cd);
     EV();
@@ -977,7 +1058,7 @@ busseq_status validate(const busseq_config* cfg)
             return BUSSEQ_ERR_ARG;
         }
     if (cfg->K < 2 || cfg->K > BQ_KMAX) { bq_set_error("K must be between 2 and 16"); return BUSSEQ_ERR_ARG; }
-    if (cfg->G < 1) { bq_set_error("G must be positive"); return BUSSEQ_ERR_ARG; }
+    if (cfg->G < 1 || cfg->G > 65535) { bq_set_error("G must be between 1 and 65535"); return BUSSEQ_ERR_ARG; }
     if (cfg->n_iter < 0 || cfg->n_burnin < 0) { bq_set_error("negative iteration count"); return BUSSEQ_ERR_ARG; }
     if (cfg->precision != 32 && cfg->precision != 64) { bq_set_error("precision must be 32 or 64"); return BUSSEQ_ERR_ARG; }
     if (cfg->chains < 1) { bq_set_error("chains must be >= 1"); return BUSSEQ_ERR_ARG; }
@@ -1026,7 +1107,11 @@ extern "C" busseq_status busseq_create(const busseq_config* cfg, const int32_t* 
     int32_t* d_batch = s->dalloc<int32_t>(N);
     int32_t* d_boff = s->dalloc<int32_t>(B + 1);
     int64_t* d_hist = s->dalloc<int64_t>((size_t)B * H);
-    if (!d_batch || !d_boff || !d_hist) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
+    int32_t* d_zrow = s->dalloc<int32_t>((size_t)N + 1);
+    int32_t* d_big_off = s->dalloc<int32_t>((size_t)B + 1);
+    uint16_t* d_zcol = nullptr;
+    int32_t* d_big = nullptr;
+    if (!d_batch || !d_boff || !d_hist || !d_zrow || !d_big_off) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
     if (h2d(d_batch, batch) != hipSuccess || h2d(d_boff, boff) != hipSuccess) {
         bq_set_error("upload failed");
         return fail(BUSSEQ_ERR_DEVICE);
@@ -1036,9 +1121,17 @@ extern "C" busseq_status busseq_create(const busseq_config* cfg, const int32_t* 
         HostInit hi;
         uint32_t key = (uint32_t)(cfg->chain0 + ch);
         if (bq_host_init(B, s->n_b.data(), G, Gp, K, Y, cfg->seed, key, hi) != 0) return fail(BUSSEQ_ERR_DATA);
-        if (ch == 0 && hipMemcpy(d_hist, hi.hist_s.data(), hi.hist_s.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
-            bq_set_error("upload failed");
-            return fail(BUSSEQ_ERR_DEVICE);
+        if (ch == 0) {
+            d_zcol = s->dalloc<uint16_t>(hi.zcol.size());
+            d_big = s->dalloc<int32_t>(hi.big.size());
+            if (!d_zcol || !d_big) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
+            if (hipMemcpy(d_hist, hi.hist_s.data(), hi.hist_s.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                h2d(d_zrow, hi.zrow) != hipSuccess || h2d(d_big_off, hi.big_off) != hipSuccess ||
+                (!hi.zcol.empty() && h2d(d_zcol, hi.zcol) != hipSuccess) ||
+                (!hi.big.empty() && h2d(d_big, hi.big) != hipSuccess)) {
+                bq_set_error("upload failed");
+                return fail(BUSSEQ_ERR_DEVICE);
+            }
         }
         ChainDev c{};
         c.N = N; c.G = G; c.Gp = Gp; c.K = K; c.B = B;
@@ -1047,6 +1140,7 @@ extern "C" busseq_status busseq_create(const busseq_config* cfg, const int32_t* 
         c.n_pres = P; c.traces = cfg->keep_traces ? 1 : 0;
         c.TR = TR; c.T_max = s->T_max; c.prec = s->prec;
         c.batch = d_batch; c.boff = d_boff; c.hist_s = d_hist;
+        c.zrow = d_zrow; c.zcol = d_zcol; c.big_off = d_big_off; c.big = d_big;
         c.iter = s->dalloc<int>(1);
         c.S = s->dalloc<int32_t>((size_t)N * Gp);
         c.W = s->dalloc<int32_t>(N);
@@ -1074,8 +1168,7 @@ extern "C" busseq_status busseq_create(const busseq_config* cfg, const int32_t* 
         c.part = s->dalloc<uint64_t>((size_t)s->T_max * 3 * Gp);
         c.hist_d = s->dalloc<uint32_t>((size_t)B * H);
         c.gstat = s->dalloc<uint64_t>((size_t)B * 5);
-        c.lsum = s->dalloc<int32_t>(G); c.bsq = s->dalloc<uint64_t>(G);
-        c.beta_acc = s->dalloc<int32_t>(G);
+        c.beta_acc = s->dalloc<int32_t>((size_t)K * G);
         c.n_rec = s->dalloc<int>(1);
         c.s_gamma = s->dalloc<double>(2 * (size_t)B); c.s_alpha = s->dalloc<double>(G);
         c.s_beta = s->dalloc<double>((size_t)K * G); c.s_nu = s->dalloc<double>((size_t)B * G);

@@ -48,6 +48,10 @@ int bq_host_init(int B, const int* n_b, int G, int Gp, int K, const int32_t* Y,
 
     o.S.assign((size_t)N * Gp, 0);
     o.hist_s.assign((size_t)B * H, 0);
+    o.zrow.assign((size_t)N + 1, 0);
+    o.zcol.clear();
+    o.big_off.assign((size_t)B + 1, 0);
+    o.big.clear();
     std::vector<int64_t> lib(N, 0);
     for (int i = 0; i < N; ++i) {
         int64_t s = 0;
@@ -57,13 +61,17 @@ int bq_host_init(int B, const int* n_b, int G, int Gp, int K, const int32_t* Y,
             o.S[(size_t)i * Gp + g] = v;
             s += v;
             if (v > 0 && v < H) o.hist_s[(size_t)batch[i] * H + v] += 1;
+            if (v == 0) o.zcol.push_back((uint16_t)g);
+            if (v >= H) { o.big.push_back(v); o.big_off[(size_t)batch[i] + 1] += 1; }
         }
+        o.zrow[(size_t)i + 1] = (int32_t)o.zcol.size();
         lib[i] = s;
         if (s <= 0) {
             bq_set_error("cell " + std::to_string(i) + " has no counts (zero library size)");
             return -1;
         }
     }
+    for (int b = 0; b < B; ++b) o.big_off[(size_t)b + 1] += o.big_off[b];
     // pi0_bk = (k+1)/(K(K+1)/2)  (cu:392-397)
     o.pi.assign((size_t)B * K, 0.0);
     for (int b = 0; b < B; ++b)

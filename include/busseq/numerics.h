@@ -309,33 +309,35 @@ BQ_FN float bq_split32(float x, int* e)
     return bq_bitsf(i - ((uint32_t)k << 23));
 }
 
+/* branch-free: every lane runs the same instructions; special inputs are
+ * patched by selects at the end (x <= 0, +inf, NaN; subnormals rescaled) */
 BQ_FN float bq_logf(float x)
 {
-    if (!(x > 0.0f)) return x == 0.0f ? -BQ_INFF : BQ_NANF;
-    if (x == BQ_INFF) return BQ_INFF;
-    int eadj = 0;
-    if (x < 1.17549435e-38f) { x *= 33554432.0f; eadj = -25; } /* 2^25 */
+    int sub = x < 1.17549435e-38f;
+    float xs = sub ? x * 33554432.0f : x; /* 2^25 */
     int e;
-    float m = bq_split32(x, &e);
+    float m = bq_split32(xs, &e);
     float f = m - 1.0f;
     float r = f + (f * f) * bq_log1p_q32(f);
-    float ef = (float)(e + eadj);
-    return ef * BQ_LN2_HI_F + (ef * BQ_LN2_LO_F + r);
+    float ef = (float)(e + (sub ? -25 : 0));
+    float res = ef * BQ_LN2_HI_F + (ef * BQ_LN2_LO_F + r);
+    if (!(x > 0.0f)) res = x == 0.0f ? -BQ_INFF : BQ_NANF;
+    if (x == BQ_INFF) res = BQ_INFF;
+    return res;
 }
 
 BQ_FN float bq_log1pf(float y)
 {
-    int e = 0;
-    float f = y;
-    if (!(y >= BQ_LOG1P_LO_F && y <= BQ_LOG1P_HI_F)) {
-        if (!(y > -1.0f)) return y == -1.0f ? -BQ_INFF : BQ_NANF;
-        if (y == BQ_INFF) return BQ_INFF;
-        float m = bq_split32(1.0f + y, &e);
-        f = m - 1.0f;
-    }
+    int inr = y >= BQ_LOG1P_LO_F && y <= BQ_LOG1P_HI_F;
+    int e;
+    float m = bq_split32(1.0f + y, &e);
+    float f = inr ? y : m - 1.0f;
+    float ef = inr ? 0.0f : (float)e;
     float r = f + (f * f) * bq_log1p_q32(f);
-    float ef = (float)e;
-    return ef * BQ_LN2_HI_F + (ef * BQ_LN2_LO_F + r);
+    float res = ef * BQ_LN2_HI_F + (ef * BQ_LN2_LO_F + r);
+    if (!(y > -1.0f)) res = y == -1.0f ? -BQ_INFF : BQ_NANF;
+    if (y == BQ_INFF) res = BQ_INFF;
+    return res;
 }
 
 /* Stirling remainder for w >= 4 in single precision (6 terms, abs err < 5e-10) */

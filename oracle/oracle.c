@@ -315,6 +315,7 @@ int orc_init(orc_t* o)
 /* per-element passes, generated for T = float and T = double                 */
 /* ------------------------------------------------------------------------ */
 #define LANES 64
+#define CHUNK 1024
 #define TL_ALPHA(tl) (tl)
 #define TL_BETA(o, tl) ((tl) + (o)->G)
 #define TL_NU0(o, tl) ((tl) + (o)->G + (size_t)(o)->K * (o)->G)
@@ -326,9 +327,10 @@ int orc_init(orc_t* o)
 
 /*
  * Canonical reduction structure (shared with the gfx950 kernels):
- *  - a sum over the genes of one cell is split into 64 gene classes
- *    (g mod 64); each class is summed in double precision in increasing g and
- *    the 64 class sums are added exactly in fixed point;
+ *  - a sum over the genes of one cell is split into chunks of 1024 genes
+ *    and, within a chunk, into 64 gene classes (g mod 64); each (chunk,
+ *    class) group is summed in double precision in increasing g and the
+ *    group sums are added exactly in fixed point;
  *  - a sum over cells for one gene runs over tiles: cells sorted stably by
  *    (batch, subtype, index) and cut into tiles of at most TR cells within a
  *    (batch, subtype) group; each tile is summed in double precision in that
@@ -383,26 +385,30 @@ static void pass_w_##SFX(orc_t* o, int t)                                       
     _Pragma("omp parallel for schedule(dynamic, 4)")                                       \
     for (int i = 0; i < o->N; ++i) {                                                       \
         int b = o->batch[i];                                                               \
-        double part[16][LANES];                                                            \
-        for (int k = 0; k < K; ++k) for (int l = 0; l < LANES; ++l) part[k][l] = 0.0;      \
-        for (int g = 0; g < G; ++g) {                                                      \
-            int32_t s = o->S[(size_t)i * G + g];                                           \
-            int xi = s < 0 ? ~s : s;                                                       \
-            T x = (T)xi;                                                                   \
-            size_t bg = (size_t)b * G + g;                                                 \
-            T phi = ph[bg];                                                                \
-            T mu0 = bq_mu_##SFX(ea[g], (T)1, en[bg], ed[i]);                               \
-            T r0 = bq_ratio_##SFX(mu0, phi);                                               \
-            for (int k = 1; k < K; ++k) {                                                  \
-                size_t kg = (size_t)k * G + g;                                             \
-                part[k][g & (LANES - 1)] += (double)bq_tw_##SFX(x, phi, r0, bt[kg], em[kg]); \
+        uint64_t acc[16];                                                                  \
+        for (int k = 0; k < K; ++k) acc[k] = 0;                                            \
+        for (int c0 = 0; c0 < G; c0 += CHUNK) {                                            \
+            double part[16][LANES];                                                        \
+            for (int k = 0; k < K; ++k) for (int l = 0; l < LANES; ++l) part[k][l] = 0.0;  \
+            int c1 = c0 + CHUNK < G ? c0 + CHUNK : G;                                      \
+            for (int g = c0; g < c1; ++g) {                                                \
+                int32_t s = o->S[(size_t)i * G + g];                                       \
+                int xi = s < 0 ? ~s : s;                                                   \
+                T x = (T)xi;                                                               \
+                size_t bg = (size_t)b * G + g;                                             \
+                T phi = ph[bg];                                                            \
+                T mu0 = bq_mu_##SFX(ea[g], (T)1, en[bg], ed[i]);                           \
+                T r0 = bq_ratio_##SFX(mu0, phi);                                           \
+                for (int k = 1; k < K; ++k) {                                              \
+                    size_t kg = (size_t)k * G + g;                                         \
+                    part[k][g & (LANES - 1)] += (double)bq_tw_##SFX(x, phi, r0, bt[kg], em[kg]); \
+                }                                                                          \
             }                                                                              \
+            for (int k = 1; k < K; ++k) for (int l = 0; l < LANES; ++l) acc[k] += bq_fix(part[k][l]); \
         }                                                                                  \
         double lp[16], mx = -BQ_INF;                                                       \
         for (int k = 0; k < K; ++k) {                                                      \
-            uint64_t acc = 0;                                                              \
-            if (k > 0) for (int l = 0; l < LANES; ++l) acc += bq_fix(part[k][l]);          \
-            double lk = bq_unfix(acc);                                                     \
+            double lk = bq_unfix(acc[k]);                                                  \
             o->w_logit[(size_t)i * K + k] = lk;                                            \
             lp[k] = lk + bq_log(o->pi[b * K + k]);                                         \
             if (lp[k] > mx) mx = lp[k];                                                    \
@@ -564,19 +570,22 @@ static void pass_delta_##SFX(orc_t* o, int t)                                   
         double dp = o->delta[i] + BQ_RW_SD * bq_normal(&r);                                \
         double dl = dp - o->delta[i];                                                      \
         T dd = (T)dl, cd = (T)bq_expm1(dl);                                                \
-        double part[LANES];                                                                \
-        for (int l = 0; l < LANES; ++l) part[l] = 0.0;                                     \
-        for (int g = 0; g < G; ++g) {                                                      \
-            int32_t s = o->S[(size_t)i * G + g];                                           \
-            int xi = s < 0 ? ~s : s;                                                       \
-            size_t bg = (size_t)b * G + g;                                                 \
-            T phi = ph[bg];                                                                \
-            T mu = bq_mu_##SFX(ea[g], eb[(size_t)k * G + g], en[bg], ed[i]);               \
-            T rt = bq_ratio_##SFX(mu, phi);                                                \
-            part[g & (LANES - 1)] += (double)bq_tmh_##SFX((T)xi, phi, rt, dd, cd);         \
-        }                                                                                  \
         uint64_t acc = 0;                                                                  \
-        for (int l = 0; l < LANES; ++l) acc += bq_fix(part[l]);                            \
+        for (int c0 = 0; c0 < G; c0 += CHUNK) {                                            \
+            double part[LANES];                                                            \
+            for (int l = 0; l < LANES; ++l) part[l] = 0.0;                                 \
+            int c1 = c0 + CHUNK < G ? c0 + CHUNK : G;                                      \
+            for (int g = c0; g < c1; ++g) {                                                \
+                int32_t s = o->S[(size_t)i * G + g];                                       \
+                int xi = s < 0 ? ~s : s;                                                   \
+                size_t bg = (size_t)b * G + g;                                             \
+                T phi = ph[bg];                                                            \
+                T mu = bq_mu_##SFX(ea[g], eb[(size_t)k * G + g], en[bg], ed[i]);           \
+                T rt = bq_ratio_##SFX(mu, phi);                                            \
+                part[g & (LANES - 1)] += (double)bq_tmh_##SFX((T)xi, phi, rt, dd, cd);     \
+            }                                                                              \
+            for (int l = 0; l < LANES; ++l) acc += bq_fix(part[l]);                        \
+        }                                                                                  \
         double sum = bq_unfix(acc);                                                        \
         double lr = sum + bq_lr_normal(o->delta[i], dp, o->mu_delta[i], BQ_SIGMA2_DELTA);  \
         bq_rng ra; bq_rng_init(&ra, o->seed, o->chain, (uint32_t)i, 0u, (uint32_t)t, BQ_PH_DELTA_A); \
@@ -954,16 +963,20 @@ double orc_bic(const orc_t* o, const orc_post* p)
         int b = o->batch[i];
         double ll[16], mx = -BQ_INF;
         for (int k = 0; k < K; ++k) {
-            double part[LANES];
-            for (int l = 0; l < LANES; ++l) part[l] = 0.0;
-            for (int g = 0; g < G; ++g) {
-                int32_t s = o->S[(size_t)i * G + g];
-                int y = s > 0 ? s : 0;
-                part[g & (LANES - 1)] += bq_bic_elem(p->alpha[g], p->beta[(size_t)k * G + g], p->nu[(size_t)b * G + g],
-                                       p->delta[i], p->gamma[b], p->gamma[B + b], p->phi[(size_t)b * G + g], y);
-            }
             uint64_t acc = 0;
-            for (int l = 0; l < LANES; ++l) acc += bq_fix(part[l]);
+            for (int c0 = 0; c0 < G; c0 += CHUNK) {
+                double part[LANES];
+                for (int l = 0; l < LANES; ++l) part[l] = 0.0;
+                int c1 = c0 + CHUNK < G ? c0 + CHUNK : G;
+                for (int g = c0; g < c1; ++g) {
+                    int32_t s = o->S[(size_t)i * G + g];
+                    int y = s > 0 ? s : 0;
+                    part[g & (LANES - 1)] += bq_bic_elem(p->alpha[g], p->beta[(size_t)k * G + g],
+                                           p->nu[(size_t)b * G + g], p->delta[i], p->gamma[b], p->gamma[B + b],
+                                           p->phi[(size_t)b * G + g], y);
+                }
+                for (int l = 0; l < LANES; ++l) acc += bq_fix(part[l]);
+            }
             ll[k] = bq_unfix(acc);
             if (ll[k] > mx) mx = ll[k];
         }
