@@ -34,6 +34,13 @@ out["lgdiff_y"] = np.array([float(mp.loggamma(mp.mpf(float(a)) + mp.mpf(float(b)
                             for a, b in zip(z, d)])
 p = np.concatenate([rng.uniform(0, 1, 1500), 10.0 ** rng.uniform(-300, -1, 200), [0.5, 0.025, 0.975]])
 out["ndtri_x"] = p
-out["ndtri_y"] = np.array([float(mp.sqrt(2) * mp.erfinv(2 * mp.mpf(float(v)) - 1)) for v in p])
+
+
+def ndtri(v):
+    with mp.workdps(360):   # 2p - 1 must keep the tiny tail probabilities
+        return float(mp.sqrt(2) * mp.erfinv(2 * mp.mpf(float(v)) - 1))
+
+
+out["ndtri_y"] = np.array([ndtri(v) for v in p])
 np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "math_vectors.npz"), **out)
 print({k: v.shape for k, v in out.items()})

@@ -1,0 +1,195 @@
+"""Numerics contract (include/busseq/numerics.h) pinned against published
+known-answer tests, 40-digit mpmath values (tests/golden/math_vectors.npz,
+written by tests/golden/make_math_vectors.py) and scipy distributions."""
+import os
+
+import numpy as np
+import pytest
+import scipy.stats as st
+
+from oracle import pyoracle as po
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "math_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD)
+
+
+# Random123 known-answer vectors for philox4x32-10 (kat_vectors of the Random123 distribution)
+KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", KAT)
+def test_philox_known_answers(ctr, key, expect):
+    assert po.philox(ctr, key).tolist() == expect
+
+
+def test_stream_words_follow_the_counter():
+    # the stream's words are the Philox blocks of counters (a, b, sweep, phase<<20 | j)
+    w = po.stream("u32", 8, seed=11, chain=3, a=5, b=6, sweep=7, phase=9)
+    b0 = po.philox([5, 6, 7, 9 << 20], [11, 3])
+    b1 = po.philox([5, 6, 7, (9 << 20) + 1], [11, 3])
+    assert w.astype(np.uint32).tolist() == b0.tolist() + b1.tolist()
+
+
+def test_uniform_open_interval_and_resolution():
+    u = po.stream("uniform", 200000, seed=1)
+    assert u.min() > 0.0 and u.max() < 1.0
+    assert st.kstest(u, "uniform").pvalue > 1e-3
+    frac = u * 2.0 ** 52
+    assert np.all(frac - np.floor(frac) == 0.5)   # centred in a 2^-52 cell
+
+
+def ulps(x, ref, dtype=np.float64):
+    ref = np.asarray(ref, dtype)
+    sp = np.spacing(np.maximum(np.abs(ref), np.finfo(dtype).tiny)).astype(np.float64)
+    return np.abs(np.asarray(x, np.float64) - ref.astype(np.float64)) / sp
+
+
+def test_log_vs_mpmath(gold):
+    assert ulps(po.math("log", gold["log_x"]), gold["log_y"]).max() <= 1.0
+
+
+def test_log1p_vs_mpmath(gold):
+    assert ulps(po.math("log1p", gold["log1p_x"]), gold["log1p_y"]).max() <= 2.0
+
+
+def test_exp_vs_mpmath(gold):
+    assert ulps(po.math("exp", gold["exp_x"]), gold["exp_y"]).max() <= 1.0
+
+
+def test_expm1_vs_mpmath(gold):
+    assert ulps(po.math("expm1", gold["expm1_x"]), gold["expm1_y"]).max() <= 4.0
+
+
+def test_lgamma_vs_mpmath(gold):
+    y = gold["lgamma_y"]
+    err = np.abs(po.math("lgamma", gold["lgamma_x"]) - y) / np.maximum(np.abs(y), 1.0)
+    # absolute error near the zeros at 1 and 2 comes from lgamma(z+n) - log(prod), ~3 ulp of lgamma(10)
+    assert err.max() < 1e-14
+
+
+def test_lgamma_diff_vs_mpmath(gold):
+    z, d, y = gold["lgdiff_z"], gold["lgdiff_d"], gold["lgdiff_y"]
+    e64 = np.abs(po.math("lgamma_diff64", z, d) - y) / np.maximum(np.abs(y), 1.0)
+    assert e64.max() < 1e-13
+    # single precision: arguments are first rounded to float; compare to the
+    # exact difference at the rounded arguments within a few float ulps of the scale
+    zf, df = z.astype(np.float32), d.astype(np.float32)
+    import mpmath as mp
+    ref = np.array([float(mp.loggamma(mp.mpf(float(a)) + mp.mpf(float(b))) - mp.loggamma(mp.mpf(float(a))))
+                    for a, b in zip(zf[:400], df[:400])])
+    e32 = np.abs(po.math("lgamma_diff32", zf[:400].astype(np.float64), df[:400].astype(np.float64)) - ref)
+    assert (e32 / np.maximum(np.abs(ref), 1.0)).max() < 2e-6
+
+
+def test_single_precision_logs_vs_mpmath(gold):
+    x = gold["log_x"].astype(np.float32)
+    x = x[np.isfinite(x) & (x > 0)]
+    import mpmath as mp
+    ref = np.array([float(mp.log(mp.mpf(float(v)))) for v in x])
+    assert ulps(po.math("logf", x.astype(np.float64)), ref, np.float32).max() <= 1.0
+    y = gold["log1p_x"].astype(np.float32)
+    y = y[y > -1]
+    ref = np.array([float(mp.log1p(mp.mpf(float(v)))) for v in y])
+    assert ulps(po.math("log1pf", y.astype(np.float64)), ref, np.float32).max() <= 3.0
+
+
+def test_ndtri_vs_mpmath(gold):
+    y = gold["ndtri_y"]
+    err = np.abs(po.math("ndtri", gold["ndtri_x"]) - y) / np.maximum(np.abs(y), 1.0)
+    assert err.max() < 1e-14
+
+
+def test_special_values():
+    inf = np.inf
+    r = po.math("log", [0.0, -1.0, inf, np.nan])
+    assert r[0] == -inf and np.isnan(r[1]) and r[2] == inf and np.isnan(r[3])
+    r = po.math("log1p", [-1.0, -2.0, inf])
+    assert r[0] == -inf and np.isnan(r[1]) and r[2] == inf
+    r = po.math("exp", [1000.0, -1000.0, np.nan])
+    assert r[0] == inf and r[1] == 0.0 and np.isnan(r[2])
+    r = po.math("log1pf", [-1.0, -2.0, inf, 0.0])
+    assert r[0] == -inf and np.isnan(r[1]) and r[2] == inf and r[3] == 0.0
+    r = po.math("softplus", [-800.0, 0.0, 800.0])
+    assert r[0] == 0.0 and abs(r[1] - np.log(2)) < 1e-16 and r[2] == 800.0
+
+
+def test_fixed_point_round_trip_and_clamp():
+    v = np.array([0.0, 1.0, -1.0, 2.0 ** -32, 3.0 * 2.0 ** -33, 123456.789, -1e9, np.nan, 1e30, -1e30])
+    r = po.math("fix", v)
+    assert r[0] == 0.0 and r[1] == 1.0 and r[2] == -1.0 and r[3] == 2.0 ** -32
+    assert r[4] == 2.0 ** -31          # 1.5 ulp rounds to even
+    assert abs(r[5] - 123456.789) <= 2.0 ** -33
+    assert r[6] == -1e9
+    assert r[7] == -4.6e18 * 2.0 ** -32   # NaN maps to the negative clamp
+    assert r[8] == 4.6e18 * 2.0 ** -32 and r[9] == -4.6e18 * 2.0 ** -32
+
+
+# --- samplers -------------------------------------------------------------------
+
+def draws(kind, n, p1=0.0, p2=0.0, seed=4, phase=10):
+    return po.stream(kind, n, seed=seed, phase=phase, p1=p1, p2=p2)
+
+
+def test_normal_distribution():
+    x = draws("normal", 100000)
+    assert st.kstest(x, "norm").pvalue > 1e-3
+    assert abs(x.mean()) < 0.02 and abs(x.std() - 1) < 0.01
+
+
+@pytest.mark.parametrize("a,scale", [(0.05, 1.0), (0.3, 2.0), (1.0, 1.0), (3.7, 0.5), (50.0, 0.1)])
+def test_gamma_distribution(a, scale):
+    x = draws("gamma", 40000, a, scale)
+    assert (x >= 0).all()
+    assert st.kstest(x, st.gamma(a, scale=scale).cdf).pvalue > 1e-3
+
+
+@pytest.mark.parametrize("lam", [0.3, 4.0, 9.99, 10.0, 47.0, 3000.0])
+def test_poisson_distribution(lam):
+    x = draws("poisson", 40000, lam).astype(int)
+    lo, hi = st.poisson.ppf(1e-4, lam), st.poisson.ppf(1 - 1e-4, lam)
+    edges = np.unique(np.linspace(lo, hi + 1, 30).astype(int))
+    obs = np.histogram(np.clip(x, lo, hi), bins=edges)[0]
+    p = np.diff(st.poisson.cdf(edges - 1, lam))
+    p[0] += st.poisson.cdf(lo - 1, lam)
+    p[-1] += st.poisson.sf(hi, lam)
+    exp = p / p.sum() * len(x)
+    keep = exp > 5
+    chi = ((obs[keep] - exp[keep]) ** 2 / exp[keep]).sum()
+    assert st.chi2.sf(chi, keep.sum() - 1) > 1e-3
+    assert abs(x.mean() - lam) < 5 * np.sqrt(lam / len(x))
+
+
+@pytest.mark.parametrize("mu,phi", [(0.5, 5.0), (20.0, 5.0), (300.0, 0.7), (5.0, 0.05)])
+def test_negative_binomial_distribution(mu, phi):
+    x = draws("nbinom", 40000, mu, phi).astype(int)
+    dist = st.nbinom(phi, phi / (phi + mu))
+    assert abs(x.mean() - mu) < 6 * np.sqrt(dist.var() / len(x))
+    hi = int(dist.ppf(0.999))
+    obs = np.bincount(np.minimum(x, hi), minlength=hi + 1)
+    p = dist.pmf(np.arange(hi + 1))
+    p[-1] += dist.sf(hi)
+    exp = p * len(x)
+    keep = exp > 5
+    chi = ((obs[keep] - exp[keep]) ** 2 / exp[keep]).sum()
+    assert st.chi2.sf(chi, keep.sum() - 1) > 1e-3
+
+
+@pytest.mark.parametrize("a,b", [(2.0, 5.0), (1.0, 3.0), (40.0, 3000.0)])
+def test_beta_distribution(a, b):
+    x = draws("beta", 30000, a, b)
+    assert st.kstest(x, st.beta(a, b).cdf).pvalue > 1e-3
+
+
+def test_planted_mistake_is_detected():
+    # a shifted normal (a plausible sign/offset slip) must fail the same KS test
+    x = draws("normal", 100000) + 0.02
+    assert st.kstest(x, "norm").pvalue < 1e-3
