@@ -42,6 +42,10 @@ _PI = _C.POINTER(_C.c_int32)
 _PD = _C.POINTER(_C.c_double)
 
 
+class _Problem(_C.Structure):
+    _fields_ = [("cfg", _C.POINTER(_Config)), ("Y", _PI)]
+
+
 class _State(_C.Structure):
     _fields_ = [("X", _PI), ("Z", _PI), ("W", _PI), ("alpha", _PD), ("beta", _PD), ("L", _PI),
                 ("nu", _PD), ("delta", _PD), ("phi", _PD), ("gamma", _PD), ("pi", _PD), ("ptau", _PD)]
@@ -85,6 +89,8 @@ def _load():
     lib.busseq_result_free.restype = None
     lib.busseq_read_dims.argtypes = [_C.c_char_p, _C.c_int, _C.POINTER(_C.c_int)]
     lib.busseq_read_counts.argtypes = [_C.c_char_p, _C.c_int, _C.c_int, _PI]
+    lib.busseq_create_batch.argtypes = [_C.POINTER(_Problem), _C.c_int, _C.POINTER(vp)]
+    lib.busseq_chain_count.argtypes = [vp]
     lib.busseq_kernel_count.restype = _C.c_int
     lib.busseq_kernel_name.argtypes = [_C.c_int]
     lib.busseq_kernel_name.restype = _C.c_char_p
@@ -197,21 +203,45 @@ def _result_to_dict(r):
 
 
 class Sampler:
-    """One libbusseq sampler handle (one or more chains on the same counts)."""
+    """One libbusseq sampler handle: one problem with one or more chains, or
+    (Sampler.batch) several problems advanced together in the same launches."""
 
     def __init__(self, Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500,
                  keep_traces=False, precision=32, chains=1, chain0=0, tile_rows=64, device=0):
+        self._init([dict(Y=Y, n_b=n_b, K=K, seed=seed, n_iter=n_iter, n_burnin=n_burnin,
+                         n_unchanged=n_unchanged, keep_traces=keep_traces, chains=chains, chain0=chain0)],
+                   precision, tile_rows, device)
+
+    @classmethod
+    def batch(cls, problems, precision=32, tile_rows=64, device=0):
+        """problems: list of dicts with Y, n_b, K and optional seed, chains, chain0,
+        n_iter, n_burnin, n_unchanged, keep_traces."""
+        obj = cls.__new__(cls)
+        obj._init(problems, precision, tile_rows, device)
+        return obj
+
+    def _init(self, problems, precision, tile_rows, device):
         lib = _load()
-        Y = _np.ascontiguousarray(Y, dtype=_np.int32)
-        self.N, self.G = Y.shape
-        self.K, self.B, self.n_b, self.chains = int(K), len(n_b), list(n_b), int(chains)
-        if sum(n_b) != self.N:
-            raise BusseqError(1, "sum(n_b) must equal the number of rows of Y")
-        cfg, nb = _config(n_b, self.G, K, seed, n_iter, n_burnin, n_unchanged, keep_traces,
-                          precision, chains, chain0, tile_rows, device)
+        self._lib, self._h = lib, None
+        keep, probs, dims = [], (_Problem * len(problems))(), []
+        for j, p in enumerate(problems):
+            Y = _np.ascontiguousarray(p["Y"], dtype=_np.int32)
+            N, G = Y.shape
+            n_b = list(p["n_b"])
+            if sum(n_b) != N:
+                raise BusseqError(1, "sum(n_b) must equal the number of rows of Y")
+            cfg, nb = _config(n_b, G, p["K"], p.get("seed", 13579), p.get("n_iter", 4000), p.get("n_burnin", 2000),
+                              p.get("n_unchanged", 500), p.get("keep_traces", False), precision,
+                              p.get("chains", 1), p.get("chain0", 0), tile_rows, device)
+            keep += [Y, cfg, nb]
+            probs[j].cfg = _C.pointer(cfg)
+            probs[j].Y = _ip(Y)
+            dims += [(N, G, int(p["K"]), len(n_b), n_b)] * int(p.get("chains", 1))
         h = _C.c_void_p()
-        _check(lib.busseq_create(_C.byref(cfg), _ip(Y), _C.byref(h)))
-        self._h, self._lib, self._keep = h, lib, nb
+        _check(lib.busseq_create_batch(probs, len(problems), _C.byref(h)))
+        self._h, self._keep, self._dims = h, keep, dims
+        self.N, self.G, self.K, self.B, self.n_b = dims[0]
+        self.chains = len(dims)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -247,7 +277,7 @@ class Sampler:
         return dict(zip(names, ms.tolist()))
 
     def state(self, chain=0):
-        N, G, K, B = self.N, self.G, self.K, self.B
+        N, G, K, B, _ = self._dims[chain]
         s = {"X": _np.zeros((N, G), _np.int32), "Z": _np.zeros((N, G), _np.int32),
              "W": _np.zeros(N, _np.int32), "alpha": _np.zeros(G), "beta": _np.zeros((K, G)),
              "L": _np.zeros((K, G), _np.int32), "nu": _np.zeros((B, G)), "delta": _np.zeros(N),

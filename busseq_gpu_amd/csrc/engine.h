@@ -58,6 +58,7 @@ struct ChainDev {
     // gamma statistics
     uint32_t* hist_d;              // [B*H] dynamic histogram (entries with Y = 0)
     uint64_t* gstat;               // [B*5] n_z1, sum x z, over0, over1_rej, over1_acc
+    double* sp_tab;                // [B*H] log(1 + e^{-(gamma0 + gamma1 x)}) for the current gamma
 
     int32_t* beta_acc;             // [K*G] 1 where this sweep accepted beta_kg
 

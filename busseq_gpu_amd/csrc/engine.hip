@@ -96,7 +96,8 @@ __device__ __forceinline__ double ph_uniform(const ChainDev& c, uint32_t a, uint
 }
 
 // ---------------------------------------------------------------------------
-// k_prep: alpha and gamma proposals (cu:514-522, 602-606), counters to zero
+// k_prep: alpha and gamma proposals (cu:514-522, 602-606), counters to zero,
+// table of the X/Z acceptance terms log(1 + e^{-(gamma0 + gamma1 x)}), x < H
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
@@ -124,7 +125,11 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
         c.gamma_p[B + j] = -bq_rgamma(&r1, -10.0 * c.gamma[B + j], 0.1);
         for (int q = 0; q < 5; ++q) c.gstat[j * 5 + q] = 0;
     }
-    if (j < c.B * H) c.hist_d[j] = 0u;
+    if (j < c.B * H) {
+        c.hist_d[j] = 0u;
+        const int b = j / H, x = j - b * H;
+        c.sp_tab[j] = bq_neg_logit_sp(c.gamma[b], c.gamma[c.B + b], (double)x);
+    }
     if (j < c.B * c.K) c.cnt[j] = 0;
 }
 
@@ -314,6 +319,7 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
     const T* eb = (const T*)c.eb + (size_t)k * G;
     const T* en = (const T*)c.en + (size_t)b * G;
     const double* phi = c.phi + (size_t)b * G;
+    const double* spt = c.sp_tab + (size_t)b * H;
     int32_t* row = c.S + (size_t)i * c.Gp;
     const int z0 = c.zrow[i], z1 = c.zrow[i + 1];
     uint64_t n1 = 0, sx1 = 0, o0 = 0, o1r = 0, o1a = 0;
@@ -328,7 +334,9 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
         int32_t sn = 0;
         if (z) {
             int xp = bq_rnbinom(&r, (double)mu, phi[g]);
-            double lr = bq_neg_logit_sp(g0, g1, (double)xo) - bq_neg_logit_sp(g0, g1, (double)xp);
+            double lo = xo < H ? spt[xo] : bq_neg_logit_sp(g0, g1, (double)xo);
+            double lp = xp < H ? spt[xp] : bq_neg_logit_sp(g0, g1, (double)xp);
+            double lr = lo - lp;
             int xi = bq_log(bq_uniform(&r)) <= lr ? xp : xo;
             sn = ~xi;
         }
@@ -886,7 +894,6 @@ __global__ __launch_bounds__(256) void k_bic(const ChainDev* __restrict__ cds, c
     a = wave_sum_u64(a);
     if (lane == 0) ll[(size_t)i * K + k] = bq_unfix(a);
 }
-
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
@@ -899,19 +906,26 @@ __global__ __launch_bounds__(256) void k_bic(const ChainDev* __restrict__ cds, c
         }                                                                             \
     } while (0)
 
-struct busseq_sampler {
-    busseq_config cfg{};
+// host-side description of one chain
+struct ChainHost {
+    int problem;
     std::vector<int> n_b;
-    int N = 0, G = 0, Gp = 0, K = 0, B = 0, C = 1, T_max = 0;
-    int prec = 32;
-    hipStream_t stream = nullptr;
+    int N, G, Gp, K, B;
+};
+
+struct busseq_sampler {
+    int C = 0;                     // chains
+    int prec = 32, TR = 64, dev = 0;
+    int maxN = 0, maxG = 0, maxK = 0, maxB = 0, maxT = 0, maxPrep = 0, maxItems = 0, maxRec = 0;
+    std::vector<ChainHost> ch;
     std::vector<ChainDev> hd;
     ChainDev* d_cd = nullptr;
-    std::vector<void*> allocs;
+    std::vector<std::pair<int, int>> krange;   // runs of consecutive chains with equal K
+    hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    std::vector<void*> allocs;
     int iter = 0;
-    int dev = 0;
 
     template <typename V>
     V* dalloc(size_t n)
@@ -932,49 +946,43 @@ static const char* const kKernelNames[] = {
     "k_b_dec", "k_n_dec", "k_delta", "k_phi", "k_phi_dec", "k_record", "k_tick"};
 static const int kNumKernels = 15;
 
-// ev: optional array of kNumKernels + 1 events recorded around each launch
+inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+// one sweep of every chain; ev: optional kNumKernels + 1 events around the launches
 template <typename T>
 busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
 {
     const dim3 blk(256);
     const int C = s->C;
-    const int N = s->N, G = s->G, B = s->B, K = s->K;
-    int prep = G;
-    if (B * H > prep) prep = B * H;
-    if (B * K > prep) prep = B * K;
-    const dim3 g_prep((prep + 255) / 256, 1, C);
-    const dim3 g_rows((N + 3) / 4, 1, C);
-    const dim3 g_plan(B, 1, C);
-    const dim3 g_tile((G + 255) / 256, s->T_max, C);
-    const int n_ad = G * (1 + (K - 1) + (B - 1) + B);
-    const dim3 g_ad((n_ad + 255) / 256, 1, C);
-    const dim3 g_gk((G * (K - 1) + 255) / 256, 1, C);
-    const dim3 g_gb1((G * (B - 1) + 255) / 256, 1, C);
-    const dim3 g_gb((G * B + 255) / 256, 1, C);
-    const dim3 g_gamma(B + 1, 1, C);
-    int rec = N;
-    if (K * G > rec) rec = K * G;
-    if (B * G > rec) rec = B * G;
-    if (B * K > rec) rec = B * K;
-    if (2 * B > rec) rec = 2 * B;
-    const dim3 g_rec((rec + 255) / 256, 1, C);
     hipStream_t st = s->stream;
     const ChainDev* cd = s->d_cd;
+    const dim3 g_prep(nblk(s->maxPrep), 1, C);
+    const dim3 g_rows((s->maxN + 3) / 4, 1, C);
+    const dim3 g_plan(s->maxB, 1, C);
+    const dim3 g_tile(nblk(s->maxG), s->maxT, C);
+    const dim3 g_items(nblk(s->maxItems), 1, C);
+    const dim3 g_gamma(s->maxB + 1, 1, C);
+    const dim3 g_rec(nblk(s->maxRec), 1, C);
     int q = 0;
 #define EV() do { if (ev) HIPCHK(hipEventRecord(ev[q], st)); ++q; } while (0)
     EV(); hipLaunchKernelGGL(k_prep<T>, g_prep, blk, 0, st, cd);
-    EV(); launch_w<T>(K, g_rows, blk, st, cd);
+    EV();
+    for (const auto& kr : s->krange) {
+        int maxn = 0;
+        for (int c = kr.first; c < kr.first + kr.second; ++c) maxn = s->ch[c].N > maxn ? s->ch[c].N : maxn;
+        launch_w<T>(s->ch[kr.first].K, dim3((maxn + 3) / 4, 1, kr.second), blk, st, cd + kr.first);
+    }
     EV(); hipLaunchKernelGGL(k_plan, g_plan, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_xz<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_ad, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_items, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_beta_nu<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_b_dec<T>, g_gk, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_n_dec<T>, g_gb1, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_b_dec<T>, g_items, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_n_dec<T>, g_items, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_phi_dec<T>, g_gb, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_phi_dec<T>, g_items, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_record, g_rec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_tick, dim3(1, 1, C), dim3(64), 0, st, cd);
     EV();
@@ -1039,7 +1047,14 @@ busseq_status upload_tcopies(busseq_sampler* s, int ch, const std::vector<double
 template <typename V>
 hipError_t h2d(V* dst, const std::vector<V>& v)
 {
-    return hipMemcpy(dst, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice);
+    return v.empty() ? hipSuccess : hipMemcpy(dst, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice);
+}
+
+template <typename V>
+hipError_t d2h(std::vector<V>& v, const V* src, size_t n)
+{
+    v.resize(n);
+    return hipMemcpy(v.data(), src, n * sizeof(V), hipMemcpyDeviceToHost);
 }
 
 busseq_status validate(const busseq_config* cfg)
@@ -1064,139 +1079,161 @@ busseq_status validate(const busseq_config* cfg)
     if (cfg->chains < 1) { bq_set_error("chains must be >= 1"); return BUSSEQ_ERR_ARG; }
     long long N = 0;
     for (int b = 0; b < cfg->B; ++b) N += cfg->n_b[b];
-    if (N * (long long)cfg->G > 2147483647LL) { bq_set_error("N x G must stay below 2^31"); return BUSSEQ_ERR_ARG; }
+    if (N * (long long)((cfg->G + 63) & ~63) > 2147483647LL) { bq_set_error("N x G must stay below 2^31"); return BUSSEQ_ERR_ARG; }
     return BUSSEQ_OK;
 }
 
+// static data of one problem, shared by its chains
+struct ProblemDev {
+    int32_t *batch, *boff, *zrow, *big_off, *big;
+    int64_t* hist;
+    uint16_t* zcol;
+};
+
 } // namespace
 
-extern "C" busseq_status busseq_create(const busseq_config* cfg, const int32_t* Y, busseq_sampler** out)
+extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_probs, busseq_sampler** out)
 {
-    if (!out) { bq_set_error("null output handle"); return BUSSEQ_ERR_ARG; }
+    if (!out || !probs || n_probs < 1) { bq_set_error("bad arguments to busseq_create_batch"); return BUSSEQ_ERR_ARG; }
     *out = nullptr;
-    busseq_status rc = validate(cfg);
-    if (rc != BUSSEQ_OK) return rc;
-    if (!Y) { bq_set_error("null count matrix"); return BUSSEQ_ERR_ARG; }
+    for (int p = 0; p < n_probs; ++p) {
+        busseq_status rc = validate(probs[p].cfg);
+        if (rc != BUSSEQ_OK) return rc;
+        if (!probs[p].Y) { bq_set_error("null count matrix"); return BUSSEQ_ERR_ARG; }
+        if (probs[p].cfg->precision != probs[0].cfg->precision || probs[p].cfg->tile_rows != probs[0].cfg->tile_rows ||
+            probs[p].cfg->device != probs[0].cfg->device) {
+            bq_set_error("all problems of a batch must share precision, tile_rows and device");
+            return BUSSEQ_ERR_ARG;
+        }
+    }
     auto* s = new busseq_sampler();
-    s->cfg = *cfg;
-    s->n_b.assign(cfg->n_b, cfg->n_b + cfg->B);
-    s->cfg.n_b = s->n_b.data();
-    s->B = cfg->B; s->G = cfg->G; s->K = cfg->K;
-    s->Gp = (cfg->G + 63) & ~63;
-    s->N = 0;
-    for (int b = 0; b < s->B; ++b) s->N += s->n_b[b];
-    s->C = cfg->chains;
-    s->prec = cfg->precision;
-    s->dev = cfg->device;
-    const int TR = cfg->tile_rows > 0 ? cfg->tile_rows : 64;
-    s->cfg.tile_rows = TR;
-    s->T_max = (s->N + TR - 1) / TR + s->B * s->K;
-    const int N = s->N, G = s->G, Gp = s->Gp, K = s->K, B = s->B;
-    const int P = cfg->n_iter - cfg->n_burnin;
     auto fail = [&](busseq_status r) { busseq_destroy(s); return r; };
+    const busseq_config* c0 = probs[0].cfg;
+    s->prec = c0->precision;
+    s->TR = c0->tile_rows > 0 ? c0->tile_rows : 64;
+    s->dev = c0->device;
     if (hipSetDevice(s->dev) != hipSuccess) { bq_set_error("hipSetDevice failed (no GPU?)"); return fail(BUSSEQ_ERR_DEVICE); }
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         bq_set_error("hipStreamCreate failed");
         return fail(BUSSEQ_ERR_DEVICE);
     }
     const size_t ts = s->prec == 32 ? 4 : 8;
-    // static data shared by the chains
-    std::vector<int32_t> batch(N), boff(B + 1, 0);
-    for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + s->n_b[b];
-    for (int b = 0; b < B; ++b) for (int i = boff[b]; i < boff[b + 1]; ++i) batch[i] = b;
-    int32_t* d_batch = s->dalloc<int32_t>(N);
-    int32_t* d_boff = s->dalloc<int32_t>(B + 1);
-    int64_t* d_hist = s->dalloc<int64_t>((size_t)B * H);
-    int32_t* d_zrow = s->dalloc<int32_t>((size_t)N + 1);
-    int32_t* d_big_off = s->dalloc<int32_t>((size_t)B + 1);
-    uint16_t* d_zcol = nullptr;
-    int32_t* d_big = nullptr;
-    if (!d_batch || !d_boff || !d_hist || !d_zrow || !d_big_off) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
-    if (h2d(d_batch, batch) != hipSuccess || h2d(d_boff, boff) != hipSuccess) {
-        bq_set_error("upload failed");
-        return fail(BUSSEQ_ERR_DEVICE);
-    }
-    s->hd.resize(s->C);
-    for (int ch = 0; ch < s->C; ++ch) {
-        HostInit hi;
-        uint32_t key = (uint32_t)(cfg->chain0 + ch);
-        if (bq_host_init(B, s->n_b.data(), G, Gp, K, Y, cfg->seed, key, hi) != 0) return fail(BUSSEQ_ERR_DATA);
-        if (ch == 0) {
-            d_zcol = s->dalloc<uint16_t>(hi.zcol.size());
-            d_big = s->dalloc<int32_t>(hi.big.size());
-            if (!d_zcol || !d_big) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
-            if (hipMemcpy(d_hist, hi.hist_s.data(), hi.hist_s.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-                h2d(d_zrow, hi.zrow) != hipSuccess || h2d(d_big_off, hi.big_off) != hipSuccess ||
-                (!hi.zcol.empty() && h2d(d_zcol, hi.zcol) != hipSuccess) ||
-                (!hi.big.empty() && h2d(d_big, hi.big) != hipSuccess)) {
-                bq_set_error("upload failed");
+    const int TR = s->TR;
+    for (int p = 0; p < n_probs; ++p) {
+        const busseq_config* cfg = probs[p].cfg;
+        const int32_t* Y = probs[p].Y;
+        const int B = cfg->B, G = cfg->G, K = cfg->K, Gp = (G + 63) & ~63;
+        std::vector<int> n_b(cfg->n_b, cfg->n_b + B);
+        int N = 0;
+        for (int b = 0; b < B; ++b) N += n_b[b];
+        const int T_max = (N + TR - 1) / TR + B * K;
+        const int P = cfg->n_iter - cfg->n_burnin;
+        std::vector<int32_t> batch(N), boff(B + 1, 0);
+        for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + n_b[b];
+        for (int b = 0; b < B; ++b) for (int i = boff[b]; i < boff[b + 1]; ++i) batch[i] = b;
+        ProblemDev pd{};
+        for (int k = 0; k < cfg->chains; ++k) {
+            HostInit hi;
+            uint32_t key = (uint32_t)(cfg->chain0 + k);
+            if (bq_host_init(B, n_b.data(), G, Gp, K, Y, cfg->seed, key, hi) != 0) return fail(BUSSEQ_ERR_DATA);
+            if (k == 0) {
+                pd.batch = s->dalloc<int32_t>(N); pd.boff = s->dalloc<int32_t>(B + 1);
+                pd.hist = s->dalloc<int64_t>((size_t)B * H); pd.zrow = s->dalloc<int32_t>((size_t)N + 1);
+                pd.big_off = s->dalloc<int32_t>((size_t)B + 1); pd.zcol = s->dalloc<uint16_t>(hi.zcol.size());
+                pd.big = s->dalloc<int32_t>(hi.big.size());
+                if (!pd.batch || !pd.boff || !pd.hist || !pd.zrow || !pd.big_off || !pd.zcol || !pd.big) {
+                    bq_set_error("device allocation failed");
+                    return fail(BUSSEQ_ERR_NOMEM);
+                }
+                if (h2d(pd.batch, batch) != hipSuccess || h2d(pd.boff, boff) != hipSuccess ||
+                    h2d(pd.hist, hi.hist_s) != hipSuccess || h2d(pd.zrow, hi.zrow) != hipSuccess ||
+                    h2d(pd.big_off, hi.big_off) != hipSuccess || h2d(pd.zcol, hi.zcol) != hipSuccess ||
+                    h2d(pd.big, hi.big) != hipSuccess) {
+                    bq_set_error("upload failed");
+                    return fail(BUSSEQ_ERR_DEVICE);
+                }
+            }
+            ChainDev c{};
+            c.N = N; c.G = G; c.Gp = Gp; c.K = K; c.B = B;
+            c.seed = cfg->seed; c.key = key;
+            c.n_burnin = cfg->n_burnin; c.n_unchanged = cfg->n_unchanged;
+            c.n_pres = P; c.traces = cfg->keep_traces ? 1 : 0;
+            c.TR = TR; c.T_max = T_max; c.prec = s->prec;
+            c.batch = pd.batch; c.boff = pd.boff; c.hist_s = pd.hist;
+            c.zrow = pd.zrow; c.zcol = pd.zcol; c.big_off = pd.big_off; c.big = pd.big;
+            c.iter = s->dalloc<int>(1);
+            c.S = s->dalloc<int32_t>((size_t)N * Gp);
+            c.W = s->dalloc<int32_t>(N);
+            c.L = s->dalloc<int32_t>((size_t)K * G);
+            c.alpha = s->dalloc<double>(G); c.beta = s->dalloc<double>((size_t)K * G);
+            c.nu = s->dalloc<double>((size_t)B * G); c.delta = s->dalloc<double>(N);
+            c.phi = s->dalloc<double>((size_t)B * G); c.gamma = s->dalloc<double>(2 * (size_t)B);
+            c.pi = s->dalloc<double>((size_t)B * K); c.ptau = s->dalloc<double>(2);
+            c.mu_alpha = s->dalloc<double>(G); c.mu_nu = s->dalloc<double>(B); c.mu_delta = s->dalloc<double>(N);
+            c.ea = s->dalloc<char>(ts * G); c.eb = s->dalloc<char>(ts * K * G); c.em = s->dalloc<char>(ts * K * G);
+            c.bt = s->dalloc<char>(ts * K * G); c.en = s->dalloc<char>(ts * B * G); c.ed = s->dalloc<char>(ts * N);
+            c.ph = s->dalloc<char>(ts * B * G);
+            c.alpha_p = s->dalloc<double>(G); c.beta_p = s->dalloc<double>((size_t)K * G);
+            c.nu_p = s->dalloc<double>((size_t)B * G); c.phi_p = s->dalloc<double>((size_t)B * G);
+            c.gamma_p = s->dalloc<double>(2 * (size_t)B);
+            c.da = s->dalloc<char>(ts * G); c.ca = s->dalloc<char>(ts * G);
+            c.db = s->dalloc<char>(ts * K * G); c.cb = s->dalloc<char>(ts * K * G);
+            c.dn = s->dalloc<char>(ts * B * G); c.cn = s->dalloc<char>(ts * B * G);
+            c.pc = s->dalloc<char>(ts * 6 * B * G);
+            c.cnt = s->dalloc<int32_t>((size_t)B * K);
+            c.perm = s->dalloc<int32_t>(N);
+            c.tiles = s->dalloc<int32_t>((size_t)T_max * 4);
+            c.tile_off = s->dalloc<int32_t>((size_t)B * K + 1);
+            c.ntiles = s->dalloc<int32_t>(1);
+            c.part = s->dalloc<uint64_t>((size_t)T_max * 3 * Gp);
+            c.hist_d = s->dalloc<uint32_t>((size_t)B * H);
+            c.gstat = s->dalloc<uint64_t>((size_t)B * 5);
+            c.sp_tab = s->dalloc<double>((size_t)B * H);
+            c.beta_acc = s->dalloc<int32_t>((size_t)K * G);
+            c.n_rec = s->dalloc<int>(1);
+            c.s_gamma = s->dalloc<double>(2 * (size_t)B); c.s_alpha = s->dalloc<double>(G);
+            c.s_beta = s->dalloc<double>((size_t)K * G); c.s_nu = s->dalloc<double>((size_t)B * G);
+            c.s_delta = s->dalloc<double>(N); c.s_phi = s->dalloc<double>((size_t)B * G);
+            c.s_pi = s->dalloc<double>((size_t)B * K); c.s_ptau = s->dalloc<double>(2);
+            c.c_L = s->dalloc<int32_t>((size_t)K * G); c.c_W = s->dalloc<int32_t>((size_t)N * K);
+            if (c.traces && P > 0) {
+                c.tr_gamma = s->dalloc<double>(2 * (size_t)B * P); c.tr_alpha = s->dalloc<double>((size_t)G * P);
+                c.tr_beta = s->dalloc<double>((size_t)(K - 1) * G * P); c.tr_ptau = s->dalloc<double>(2 * (size_t)P);
+                c.tr_nu = s->dalloc<double>((size_t)(B - 1) * G * P); c.tr_delta = s->dalloc<double>((size_t)N * P);
+                c.tr_phi = s->dalloc<double>((size_t)B * G * P); c.tr_pi = s->dalloc<double>((size_t)B * K * P);
+                c.tr_L = s->dalloc<int32_t>((size_t)(K - 1) * G * P); c.tr_W = s->dalloc<int32_t>((size_t)N * P);
+                if (!c.tr_W || !c.tr_L || !c.tr_delta) { bq_set_error("device allocation of traces failed"); return fail(BUSSEQ_ERR_NOMEM); }
+            }
+            if (!c.part || !c.S || !c.c_W || !c.pc || !c.ptau || !c.sp_tab) {
+                bq_set_error("device allocation failed");
+                return fail(BUSSEQ_ERR_NOMEM);
+            }
+            if (h2d(c.S, hi.S) != hipSuccess || h2d(c.W, hi.W) != hipSuccess || h2d(c.L, hi.L) != hipSuccess ||
+                h2d(c.alpha, hi.alpha) != hipSuccess || h2d(c.beta, hi.beta) != hipSuccess ||
+                h2d(c.nu, hi.nu) != hipSuccess || h2d(c.delta, hi.delta) != hipSuccess ||
+                h2d(c.phi, hi.phi) != hipSuccess || h2d(c.gamma, hi.gamma) != hipSuccess ||
+                h2d(c.pi, hi.pi) != hipSuccess || h2d(c.ptau, hi.ptau) != hipSuccess ||
+                h2d(c.mu_alpha, hi.mu_alpha) != hipSuccess || h2d(c.mu_nu, hi.mu_nu) != hipSuccess ||
+                h2d(c.mu_delta, hi.mu_delta) != hipSuccess) {
+                bq_set_error("upload of the initial state failed");
                 return fail(BUSSEQ_ERR_DEVICE);
             }
+            s->hd.push_back(c);
+            s->ch.push_back(ChainHost{p, n_b, N, G, Gp, K, B});
+            busseq_status rc = upload_tcopies(s, (int)s->hd.size() - 1, hi.alpha, hi.beta, hi.nu, hi.delta, hi.phi);
+            if (rc != BUSSEQ_OK) return fail(rc);
+            auto mx = [](int& a, long v) { if (v > a) a = (int)v; };
+            mx(s->maxN, N); mx(s->maxG, G); mx(s->maxK, K); mx(s->maxB, B); mx(s->maxT, T_max);
+            mx(s->maxPrep, G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * K);
+            mx(s->maxItems, (long)G * (1 + (K - 1) + (B - 1) + B));
+            mx(s->maxRec, N); mx(s->maxRec, (long)K * G); mx(s->maxRec, (long)B * G); mx(s->maxRec, (long)B * K);
+            mx(s->maxRec, 2L * B);
         }
-        ChainDev c{};
-        c.N = N; c.G = G; c.Gp = Gp; c.K = K; c.B = B;
-        c.seed = cfg->seed; c.key = key;
-        c.n_burnin = cfg->n_burnin; c.n_unchanged = cfg->n_unchanged;
-        c.n_pres = P; c.traces = cfg->keep_traces ? 1 : 0;
-        c.TR = TR; c.T_max = s->T_max; c.prec = s->prec;
-        c.batch = d_batch; c.boff = d_boff; c.hist_s = d_hist;
-        c.zrow = d_zrow; c.zcol = d_zcol; c.big_off = d_big_off; c.big = d_big;
-        c.iter = s->dalloc<int>(1);
-        c.S = s->dalloc<int32_t>((size_t)N * Gp);
-        c.W = s->dalloc<int32_t>(N);
-        c.L = s->dalloc<int32_t>((size_t)K * G);
-        c.alpha = s->dalloc<double>(G); c.beta = s->dalloc<double>((size_t)K * G);
-        c.nu = s->dalloc<double>((size_t)B * G); c.delta = s->dalloc<double>(N);
-        c.phi = s->dalloc<double>((size_t)B * G); c.gamma = s->dalloc<double>(2 * (size_t)B);
-        c.pi = s->dalloc<double>((size_t)B * K); c.ptau = s->dalloc<double>(2);
-        c.mu_alpha = s->dalloc<double>(G); c.mu_nu = s->dalloc<double>(B); c.mu_delta = s->dalloc<double>(N);
-        c.ea = s->dalloc<char>(ts * G); c.eb = s->dalloc<char>(ts * K * G); c.em = s->dalloc<char>(ts * K * G);
-        c.bt = s->dalloc<char>(ts * K * G); c.en = s->dalloc<char>(ts * B * G); c.ed = s->dalloc<char>(ts * N);
-        c.ph = s->dalloc<char>(ts * B * G);
-        c.alpha_p = s->dalloc<double>(G); c.beta_p = s->dalloc<double>((size_t)K * G);
-        c.nu_p = s->dalloc<double>((size_t)B * G); c.phi_p = s->dalloc<double>((size_t)B * G);
-        c.gamma_p = s->dalloc<double>(2 * (size_t)B);
-        c.da = s->dalloc<char>(ts * G); c.ca = s->dalloc<char>(ts * G);
-        c.db = s->dalloc<char>(ts * K * G); c.cb = s->dalloc<char>(ts * K * G);
-        c.dn = s->dalloc<char>(ts * B * G); c.cn = s->dalloc<char>(ts * B * G);
-        c.pc = s->dalloc<char>(ts * 6 * B * G);
-        c.cnt = s->dalloc<int32_t>((size_t)B * K);
-        c.perm = s->dalloc<int32_t>(N);
-        c.tiles = s->dalloc<int32_t>((size_t)s->T_max * 4);
-        c.tile_off = s->dalloc<int32_t>((size_t)B * K + 1);
-        c.ntiles = s->dalloc<int32_t>(1);
-        c.part = s->dalloc<uint64_t>((size_t)s->T_max * 3 * Gp);
-        c.hist_d = s->dalloc<uint32_t>((size_t)B * H);
-        c.gstat = s->dalloc<uint64_t>((size_t)B * 5);
-        c.beta_acc = s->dalloc<int32_t>((size_t)K * G);
-        c.n_rec = s->dalloc<int>(1);
-        c.s_gamma = s->dalloc<double>(2 * (size_t)B); c.s_alpha = s->dalloc<double>(G);
-        c.s_beta = s->dalloc<double>((size_t)K * G); c.s_nu = s->dalloc<double>((size_t)B * G);
-        c.s_delta = s->dalloc<double>(N); c.s_phi = s->dalloc<double>((size_t)B * G);
-        c.s_pi = s->dalloc<double>((size_t)B * K); c.s_ptau = s->dalloc<double>(2);
-        c.c_L = s->dalloc<int32_t>((size_t)K * G); c.c_W = s->dalloc<int32_t>((size_t)N * K);
-        if (c.traces && P > 0) {
-            c.tr_gamma = s->dalloc<double>(2 * (size_t)B * P); c.tr_alpha = s->dalloc<double>((size_t)G * P);
-            c.tr_beta = s->dalloc<double>((size_t)(K - 1) * G * P); c.tr_ptau = s->dalloc<double>(2 * (size_t)P);
-            c.tr_nu = s->dalloc<double>((size_t)(B - 1) * G * P); c.tr_delta = s->dalloc<double>((size_t)N * P);
-            c.tr_phi = s->dalloc<double>((size_t)B * G * P); c.tr_pi = s->dalloc<double>((size_t)B * K * P);
-            c.tr_L = s->dalloc<int32_t>((size_t)(K - 1) * G * P); c.tr_W = s->dalloc<int32_t>((size_t)N * P);
-            if (!c.tr_W || !c.tr_L || !c.tr_delta) { bq_set_error("device allocation of traces failed"); return fail(BUSSEQ_ERR_NOMEM); }
-        }
-        if (!c.part || !c.S || !c.c_W || !c.pc) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
-        s->hd[ch] = c;
-        if (h2d(c.S, hi.S) != hipSuccess || h2d(c.W, hi.W) != hipSuccess || h2d(c.L, hi.L) != hipSuccess ||
-            h2d(c.alpha, hi.alpha) != hipSuccess || h2d(c.beta, hi.beta) != hipSuccess ||
-            h2d(c.nu, hi.nu) != hipSuccess || h2d(c.delta, hi.delta) != hipSuccess ||
-            h2d(c.phi, hi.phi) != hipSuccess || h2d(c.gamma, hi.gamma) != hipSuccess ||
-            h2d(c.pi, hi.pi) != hipSuccess || h2d(c.ptau, hi.ptau) != hipSuccess ||
-            h2d(c.mu_alpha, hi.mu_alpha) != hipSuccess || h2d(c.mu_nu, hi.mu_nu) != hipSuccess ||
-            h2d(c.mu_delta, hi.mu_delta) != hipSuccess) {
-            bq_set_error("upload of the initial state failed");
-            return fail(BUSSEQ_ERR_DEVICE);
-        }
-        rc = upload_tcopies(s, ch, hi.alpha, hi.beta, hi.nu, hi.delta, hi.phi);
-        if (rc != BUSSEQ_OK) return fail(rc);
+    }
+    s->C = (int)s->hd.size();
+    for (int c = 0; c < s->C; ++c) {
+        if (!s->krange.empty() && s->ch[s->krange.back().first].K == s->ch[c].K) s->krange.back().second += 1;
+        else s->krange.push_back({c, 1});
     }
     if (hipMalloc(&s->d_cd, sizeof(ChainDev) * s->C) != hipSuccess) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
     if (hipMemcpy(s->d_cd, s->hd.data(), sizeof(ChainDev) * s->C, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1206,6 +1243,14 @@ extern "C" busseq_status busseq_create(const busseq_config* cfg, const int32_t* 
     if (hipDeviceSynchronize() != hipSuccess) { bq_set_error("device sync failed"); return fail(BUSSEQ_ERR_DEVICE); }
     *out = s;
     return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_create(const busseq_config* cfg, const int32_t* Y, busseq_sampler** out)
+{
+    busseq_problem p;
+    p.cfg = cfg;
+    p.Y = Y;
+    return busseq_create_batch(&p, 1, out);
 }
 
 extern "C" void busseq_destroy(busseq_sampler* s)
@@ -1222,6 +1267,7 @@ extern "C" void busseq_destroy(busseq_sampler* s)
 }
 
 extern "C" int busseq_iteration(const busseq_sampler* s) { return s ? s->iter : -1; }
+extern "C" int busseq_chain_count(const busseq_sampler* s) { return s ? s->C : 0; }
 
 extern "C" busseq_status busseq_sweeps(busseq_sampler* s, int n)
 {
@@ -1304,6 +1350,7 @@ extern "C" busseq_status busseq_copy_probe(size_t bytes, int device, double* ms)
     HIPCHK(hipMalloc(&a, n * 16));
     HIPCHK(hipMalloc(&b, n * 16));
     HIPCHK(hipMemset(a, 1, n * 16));
+    HIPCHK(hipMemset(b, 0, n * 16));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -1323,20 +1370,13 @@ extern "C" busseq_status busseq_copy_probe(size_t bytes, int device, double* ms)
     return BUSSEQ_OK;
 }
 
-template <typename V>
-static hipError_t d2h(std::vector<V>& v, const V* src, size_t n)
-{
-    v.resize(n);
-    return hipMemcpy(v.data(), src, n * sizeof(V), hipMemcpyDeviceToHost);
-}
-
 extern "C" busseq_status busseq_get_state(busseq_sampler* s, int ch, busseq_state* st)
 {
     if (!s || !st || ch < 0 || ch >= s->C) { bq_set_error("bad arguments to busseq_get_state"); return BUSSEQ_ERR_ARG; }
     HIPCHK(hipSetDevice(s->dev));
     HIPCHK(hipStreamSynchronize(s->stream));
     const ChainDev& c = s->hd[ch];
-    const int N = s->N, G = s->G, Gp = s->Gp, K = s->K, B = s->B;
+    const int N = c.N, G = c.G, Gp = c.Gp, K = c.K, B = c.B;
     if (st->X || st->Z) {
         std::vector<int32_t> S;
         HIPCHK(d2h(S, c.S, (size_t)N * Gp));
@@ -1366,7 +1406,7 @@ extern "C" busseq_status busseq_set_state(busseq_sampler* s, int ch, const busse
     HIPCHK(hipSetDevice(s->dev));
     HIPCHK(hipStreamSynchronize(s->stream));
     const ChainDev& c = s->hd[ch];
-    const int N = s->N, G = s->G, Gp = s->Gp, K = s->K, B = s->B;
+    const int N = c.N, G = c.G, Gp = c.Gp, K = c.K, B = c.B;
     if (st->X && st->Z) {
         std::vector<int32_t> S;
         HIPCHK(d2h(S, c.S, (size_t)N * Gp));
@@ -1408,7 +1448,8 @@ extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result*
     HIPCHK(hipStreamSynchronize(s->stream));
     std::memset(r, 0, sizeof(*r));
     const ChainDev& c = s->hd[ch];
-    const int N = s->N, G = s->G, K = s->K, B = s->B;
+    const ChainHost& h = s->ch[ch];
+    const int N = c.N, G = c.G, K = c.K, B = c.B;
     int nrec = 0;
     HIPCHK(hipMemcpy(&nrec, c.n_rec, 4, hipMemcpyDeviceToHost));
     if (nrec <= 0) { bq_set_error("no recorded iterations (n_iter must exceed n_burnin)"); return BUSSEQ_ERR_ARG; }
@@ -1418,7 +1459,7 @@ extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result*
     r->B = B; r->N = N; r->G = G; r->K = K;
     r->n_pres = c.traces ? (c.n_pres < nrec ? c.n_pres : nrec) : 0;
     r->n_b = (int*)std::calloc(B, sizeof(int));
-    for (int b = 0; b < B; ++b) r->n_b[b] = s->n_b[b];
+    for (int b = 0; b < B; ++b) r->n_b[b] = h.n_b[b];
     r->X = mi((size_t)N * G); r->Z = mi((size_t)N * G); r->W = mi(N);
     r->gamma = md(2 * (size_t)B); r->alpha = md(G); r->L = md((size_t)K * G); r->beta = md((size_t)K * G);
     r->ptau = md(2); r->nu = md((size_t)B * G); r->delta = md(N); r->phi = md((size_t)B * G); r->pi = md((size_t)B * K);
@@ -1467,7 +1508,7 @@ extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result*
         (void)hipFree(d_pm);
         (void)hipFree(d_ll);
         std::vector<int> batch(N);
-        { int i = 0; for (int b = 0; b < B; ++b) for (int q = 0; q < s->n_b[b]; ++q) batch[i++] = b; }
+        { int i = 0; for (int b = 0; b < B; ++b) for (int q = 0; q < h.n_b[b]; ++q) batch[i++] = b; }
         double tot = 0.0;
         for (int i = 0; i < N; ++i) {
             int b = batch[i];

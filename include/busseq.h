@@ -102,6 +102,20 @@ busseq_status busseq_set_state(busseq_sampler* s, int chain, const busseq_state*
 busseq_status busseq_finish(busseq_sampler* s, int chain, busseq_result* out);
 void busseq_destroy(busseq_sampler* s);
 
+/* Batched form (no counterpart in the reference, which runs one chain per
+ * process): several problems -- different count matrices, K values, seeds --
+ * each with cfg->chains chains, advanced together: every sweep of every chain
+ * runs in the same kernel launches of one hipGraph.  Chains are numbered in
+ * problem order; chain j of a problem uses Philox key word cfg->chain0 + j and
+ * reproduces the plain run with that chain0 bit for bit.  Problems must share
+ * precision, tile_rows and device. */
+typedef struct busseq_problem {
+    const busseq_config* cfg;
+    const int32_t* Y;
+} busseq_problem;
+busseq_status busseq_create_batch(const busseq_problem* probs, int n_probs, busseq_sampler** out);
+int busseq_chain_count(const busseq_sampler* s);
+
 /* Input files.  -N file: B whitespace-separated ints (cu:1094-1098).
  * -c file: N*G whitespace-separated ints, cells as rows (cu:1145-1159). */
 busseq_status busseq_read_dims(const char* path, int B, int* n_b);

@@ -462,6 +462,23 @@ BQ_FN double bq_rgamma(bq_rng* r, double a, double scale)
     return g * scale;
 }
 
+/* 1/k for k = 0..255 (entry 0 unused), correctly rounded doubles */
+#ifdef __HIPCC__
+#define BQ_TABLE static __constant__ const
+#else
+#define BQ_TABLE static const
+#endif
+#define BQ_INV_N 256
+BQ_TABLE double bq_inv_tab[BQ_INV_N] = {
+#define BQ_I4(a) 1.0 / (a), 1.0 / (a + 1), 1.0 / (a + 2), 1.0 / (a + 3)
+#define BQ_I16(a) BQ_I4(a), BQ_I4(a + 4), BQ_I4(a + 8), BQ_I4(a + 12)
+    0.0, 1.0, 1.0 / 2, 1.0 / 3, BQ_I4(4), BQ_I4(8), BQ_I4(12),
+    BQ_I16(16), BQ_I16(32), BQ_I16(48), BQ_I16(64), BQ_I16(80), BQ_I16(96), BQ_I16(112),
+    BQ_I16(128), BQ_I16(144), BQ_I16(160), BQ_I16(176), BQ_I16(192), BQ_I16(208), BQ_I16(224), BQ_I16(240)
+#undef BQ_I16
+#undef BQ_I4
+};
+
 /* Poisson(lam): inversion for lam < 10, Hoermann (1993) PTRS otherwise */
 BQ_FN int bq_rpois(bq_rng* r, double lam)
 {
@@ -471,7 +488,7 @@ BQ_FN int bq_rpois(bq_rng* r, double lam)
         double s = p;
         double u = bq_uniform(r);
         int k = 0;
-        while (u > s && k < 200) { k += 1; p *= lam / (double)k; s += p; }
+        while (u > s && k < 200) { k += 1; p *= lam * bq_inv_tab[k]; s += p; }
         return k;
     }
     if (lam > 1.0e9) lam = 1.0e9;
