@@ -297,17 +297,36 @@ __global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
 // ---------------------------------------------------------------------------
 // k_xz: X/Z imputation (cu:484-512) of the Y = 0 entries, one wave per cell,
 // lanes over the cell's static list of zero entries; dynamic gamma histogram.
+// The Z draw runs on every entry; entries with Z = 1 are queued in LDS with
+// their random-stream state and their negative-binomial proposals are drawn
+// 64 at a time, so the long double-precision path runs with full lanes.
 // An entry is written back only when its state changes.
 // ---------------------------------------------------------------------------
+struct XzItem {
+    int g;
+    int32_t s;
+    double mu;
+    uint32_t b0, b1, b2;
+    int n;
+};
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
 {
     const ChainDev& c = cds[blockIdx.z];
     __shared__ uint32_t hist[4][H];
+    __shared__ XzItem queue[4][128];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 4 + wv;
     if (i >= c.N) return;
     uint32_t* hw = hist[wv];
+    XzItem* qw = queue[wv];
     for (int x = lane; x < H; x += 64) hw[x] = 0u;
     const int G = c.G, B = c.B, t = *c.iter;
     const int b = c.batch[i], k = c.W[i];
@@ -322,43 +341,79 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
     const double* spt = c.sp_tab + (size_t)b * H;
     int32_t* row = c.S + (size_t)i * c.Gp;
     const int z0 = c.zrow[i], z1 = c.zrow[i + 1];
+    const uint64_t below = (1ull << lane) - 1ull;
     uint64_t n1 = 0, sx1 = 0, o0 = 0, o1r = 0, o1a = 0;
-    for (int p = z0 + lane; p < z1; p += 64) {
-        const int g = c.zcol[p];
-        const int32_t s = row[g];
-        const T mu = tmu(ea[g], eb[g], en[g], edi);
-        bq_rng r;
-        bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ);
-        int xo = s < 0 ? ~s : 0;
-        int z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
-        int32_t sn = 0;
-        if (z) {
-            int xp = bq_rnbinom(&r, (double)mu, phi[g]);
-            double lo = xo < H ? spt[xo] : bq_neg_logit_sp(g0, g1, (double)xo);
-            double lp = xp < H ? spt[xp] : bq_neg_logit_sp(g0, g1, (double)xp);
-            double lr = lo - lp;
-            int xi = bq_log(bq_uniform(&r)) <= lr ? xp : xo;
-            sn = ~xi;
+    int qn = 0;
+    for (int base = z0;; base += 64) {
+        const bool more = base < z1;
+        if (more) {
+            const int p = base + lane;
+            const bool act = p < z1;
+            int g = 0, z = 0;
+            int32_t s = 0;
+            bq_rng r;
+            if (act) {
+                g = c.zcol[p];
+                s = row[g];
+                bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ);
+                z = s < 0 ? 1 : (bq_uniform(&r) > qz);
+            }
+            const uint64_t m = __ballot(act && z);
+            if (act && z) {
+                XzItem it;
+                it.g = g;
+                it.s = s;
+                it.mu = (double)tmu(ea[g], eb[g], en[g], edi);
+                it.b0 = r.b0; it.b1 = r.b1; it.b2 = r.b2; it.n = r.n;
+                qw[qn + __popcll(m & below)] = it;
+            }
+            if (act && !z) {
+                if (s != 0) row[g] = 0;
+                atomicAdd(&hw[0], 1u);
+            }
+            qn += __popcll(m);
         }
-        if (sn != s) row[g] = sn;
-        const int xi = sn < 0 ? ~sn : 0;
-        if (sn < 0) { n1 += 1; sx1 += (uint64_t)xi; }
-        if (xi < H) {
-            atomicAdd(&hw[xi], 1u);
-        } else {
-            double xd = (double)xi;
-            double a0 = bq_sp_lin(g0, g1, xd), a1 = bq_sp_lin(g0p, g1, xd);
-            double a2 = bq_sp_lin(g0, g1p, xd), a3 = bq_sp_lin(g0p, g1p, xd);
-            o0 += bq_fix(a0 - a1);
-            o1r += bq_fix(a0 - a2);
-            o1a += bq_fix(a1 - a3);
+        if (qn >= 64 || (!more && qn > 0)) {
+            const int take = qn < 64 ? qn : 64;
+            wave_lds_sync();
+            if (lane < take) {
+                const XzItem it = qw[qn - take + lane];
+                const int g = it.g;
+                const int xo = it.s < 0 ? ~it.s : 0;
+                bq_rng r;
+                r.k0 = c.seed; r.k1 = c.key; r.c0 = (uint32_t)i; r.c1 = (uint32_t)g; r.c2 = (uint32_t)t;
+                r.c3 = ((uint32_t)BQ_PH_XZ << 20) + (it.n > 0 ? 1u : 0u);
+                r.b0 = it.b0; r.b1 = it.b1; r.b2 = it.b2; r.b3 = 0u; r.n = it.n;
+                int xp = bq_rnbinom(&r, it.mu, phi[g]);
+                double lo = xo < H ? spt[xo] : bq_neg_logit_sp(g0, g1, (double)xo);
+                double lp = xp < H ? spt[xp] : bq_neg_logit_sp(g0, g1, (double)xp);
+                int xi = bq_log(bq_uniform(&r)) <= lo - lp ? xp : xo;
+                const int32_t sn = ~xi;
+                if (sn != it.s) row[g] = sn;
+                n1 += 1;
+                sx1 += (uint64_t)xi;
+                if (xi < H) {
+                    atomicAdd(&hw[xi], 1u);
+                } else {
+                    double xd = (double)xi;
+                    double a0 = bq_sp_lin(g0, g1, xd), a1 = bq_sp_lin(g0p, g1, xd);
+                    double a2 = bq_sp_lin(g0, g1p, xd), a3 = bq_sp_lin(g0p, g1p, xd);
+                    o0 += bq_fix(a0 - a1);
+                    o1r += bq_fix(a0 - a2);
+                    o1a += bq_fix(a1 - a3);
+                }
+            }
+            qn -= take;
+            wave_lds_sync();
         }
+        if (!more) break;
     }
     n1 = wave_sum_u64(n1);
     sx1 = wave_sum_u64(sx1);
     o0 = wave_sum_u64(o0);
     o1r = wave_sum_u64(o1r);
     o1a = wave_sum_u64(o1a);
+    wave_lds_sync();
     for (int x = lane; x < H; x += 64) {
         uint32_t v = hw[x];
         if (v) atomicAdd(&c.hist_d[b * H + x], v);

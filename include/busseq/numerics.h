@@ -110,13 +110,14 @@ BQ_FN uint32_t bq_next32(bq_rng* r)
     return w;
 }
 
-/* uniform on the open interval (0,1): 52 random bits, centred in their cell */
+/* uniform on the open interval (0,1) from one 32-bit word, centred in its
+ * 2^-32 cell: ((w + 1/2) 2^-32) is exact in double.  A resolution of 2^-32
+ * bounds the bias of every Bernoulli, inversion and accept/reject decision
+ * that uses it by 2^-32. */
 BQ_FN double bq_uniform(bq_rng* r)
 {
-    uint32_t w0 = bq_next32(r);
-    uint32_t w1 = bq_next32(r);
-    uint64_t j = ((uint64_t)(w0 >> 6) << 26) | (uint64_t)(w1 >> 6);
-    return ((double)j + 0.5) * 2.220446049250313080847e-16; /* 2^-52 */
+    uint32_t w = bq_next32(r);
+    return ((double)w + 0.5) * 2.3283064365386962890625e-10; /* 2^-32 */
 }
 
 /* ------------------------------------------------------------------------ */

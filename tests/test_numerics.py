@@ -43,8 +43,10 @@ def test_uniform_open_interval_and_resolution():
     u = po.stream("uniform", 200000, seed=1)
     assert u.min() > 0.0 and u.max() < 1.0
     assert st.kstest(u, "uniform").pvalue > 1e-3
-    frac = u * 2.0 ** 52
-    assert np.all(frac - np.floor(frac) == 0.5)   # centred in a 2^-52 cell
+    frac = u * 2.0 ** 32
+    assert np.all(frac - np.floor(frac) == 0.5)   # centred in a 2^-32 cell
+    w = po.stream("u32", 64, seed=1)
+    assert np.array_equal(u[:64], (w + 0.5) * 2.0 ** -32)   # one Philox word per uniform
 
 
 def ulps(x, ref, dtype=np.float64):

@@ -142,7 +142,10 @@ def test_beta_nu_sums_and_decisions(sweep):
         mu = mean(post["alpha"], post["beta"], p["nu"], p["delta"], W, b)[rows]
         dl = d["nu_prop"][bb] - p["nu"][bb]
         terms = lnb(X[rows], mu * np.exp(dl)[None], phi[rows]) - lnb(X[rows], mu, phi[rows])
-        assert np.all(np.abs(d["nu_sum"][bb] - terms.sum(0)) <= tol(s, 1 + np.abs(terms).sum(0)))
+        # the nu term given an accepted beta is a difference of two log1p of size (phi+x)|dbeta + dnu|
+        dlb = np.abs(d["beta_prop"] - p["beta"])[W[rows]]
+        comp = ((phi[rows] + X[rows]) * (dlb + np.abs(dl)[None])).sum(0)
+        assert np.all(np.abs(d["nu_sum"][bb] - terms.sum(0)) <= tol(s, 1 + comp + np.abs(terms).sum(0)))
         for g in range(s["G"]):
             m = s["pri"]["mu_nu"][bb]
             lr = d["nu_sum"][bb, g] + (st.norm.logpdf(d["nu_prop"][bb, g], m, np.sqrt(5))
