@@ -43,13 +43,14 @@ def log(*a):
 
 
 def kernel_bytes(name, N, G, n_zero):
-    """Algorithmic HBM bytes of one launch: the int32 state matrix S read once
-    (4 B per cell x gene); k_xz_alpha also writes S back for the Y = 0 entries."""
-    base = 4.0 * N * G
-    if name == "k_xz_alpha":
-        return base + 4.0 * n_zero
-    if name in ("k_w", "k_beta_nu", "k_delta", "k_phi"):
-        return base
+    """Algorithmic HBM bytes of one launch.  The streaming passes read the int32
+    state matrix S once (4 B per cell x gene).  k_xz touches only the Y = 0
+    entries: their uint16 gene index and their S word (6 B read) and writes S
+    back when it changes (at most 4 B)."""
+    if name == "k_xz":
+        return 10.0 * n_zero
+    if name in ("k_w", "k_alpha", "k_beta_nu", "k_delta", "k_phi"):
+        return 4.0 * N * G
     return None
 
 
@@ -84,6 +85,37 @@ def cpu_baseline(Y, cfg, precision, budget_s):
                       f"(oracle/oracle.c, OpenMP, {threads} threads, precision {precision})"}
 
 
+def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
+    """c4: K-sweep K=3..10 with 4 chains each (32 chains) on the c1 data in one
+    sampler (every chain in the same launches of one hipGraph) against the same
+    32 chains run one after another in the plain form."""
+    cfg = CONFIGS["c1"]
+    n_b, G = cfg["n_b"], cfg["G"]
+    Y = bq.simulate(n_b, G, cfg["K"], cfg["data_seed"], gamma0_range=cfg["g0"])
+    Ks = list(range(3, 11))
+    chains = 4
+    probs = [dict(Y=Y, n_b=n_b, K=K, seed=13579, chains=chains, n_iter=4000, n_burnin=2000, n_unchanged=500)
+             for K in Ks]
+    sb = bq.Sampler.batch(probs, precision=precision)
+    sb.sweeps(warmup)
+    ms_b = sb.time_sweeps(sweeps)
+    sb.close()
+    total_plain_ms = 0.0
+    for K in Ks:
+        for c in range(chains):
+            with bq.Sampler(Y, n_b, K, seed=13579, chain0=c, precision=precision) as sp:
+                sp.sweeps(warmup)
+                total_plain_ms += sp.time_sweeps(sweeps)
+    n_chains = len(Ks) * chains
+    cs_b = n_chains * sweeps / (ms_b / 1e3)
+    cs_p = n_chains * sweeps / (total_plain_ms / 1e3)
+    N = sum(n_b)
+    return {"config": "c4: K=3..10 x 4 chains on c1 data (N=1000, G=3000, B=4)", "chains": n_chains,
+            "sweeps": sweeps, "batched_chain_sweeps_per_s": round(cs_b, 1),
+            "plain_sequential_chain_sweeps_per_s": round(cs_p, 1), "ratio": round(cs_b / cs_p, 2),
+            "batched_cell_gene_updates_per_s": round(cs_b * N * G, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +126,7 @@ def main():
     ap.add_argument("--kernel-sweeps", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -138,6 +171,12 @@ def main():
     sweep_bytes = 16.0 * N * G
     s.close()
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_vs_plain(bq, args.precision)
+        log(f"[bench] batched c4: {batched['batched_chain_sweeps_per_s']} chain-sweeps/s vs plain sequential "
+            f"{batched['plain_sequential_chain_sweeps_per_s']} -> {batched['ratio']}x")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(Y, cfg, args.precision, args.cpu_budget)
@@ -166,6 +205,7 @@ def main():
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(sweep_bytes * sweeps_per_s / 1e9 / HBM_PEAK_GBS, 5)},
         "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
+        "batched_vs_plain": batched,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(sweeps_per_s / cpu["value"], 1) if cpu else None,
     }

@@ -356,7 +356,8 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
                 g = c.zcol[p];
                 s = row[g];
                 bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ);
-                z = s < 0 ? 1 : (bq_uniform(&r) > qz);
+                const int xo = s < 0 ? ~s : 0;   // X = 0 redraws Z whatever Z was (cu:493-496)
+                z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
             }
             const uint64_t m = __ballot(act && z);
             if (act && z) {
