@@ -1,7 +1,7 @@
 // cli.cpp -- the BUSseq_gpu executable: the reference's command line
 // (BUSseq_gpu.cu:1026-1134), progress lines and outputs, over libbusseq.
 //
-//   BUSseq_gpu -B 4 -N dim.txt -G 3000 -K 5 -s 13579 -c count.txt \
+//   BUSseq_gpu -B 4 -N dim.txt -G 3000 -K 5 -s 13579 -c count.txt
 //              -i 4000 -b 2000 -u 500 [-p] -o prefix
 //
 // Additive long options (defaults reproduce the reference's behaviour):

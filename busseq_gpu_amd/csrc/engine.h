@@ -13,63 +13,73 @@
 #define BQ_BMAX 128    // largest B (the reference's __constant__ limit, cu:9-10)
 #define BQ_WAVE 64
 #define BQ_CHUNK 1024 // genes per row-sum chunk (part of the reduction contract)
+#define BQ_HLO 64      // histogram bins kept in striped copies (the frequent small values)
+#define BQ_STRIPES 32 // copies of each per-sweep counter, one per group of workgroups, so that
+                      // the atomics of thousands of cells do not queue on one cache line
 
-struct ChainDev {
-    // dimensions and schedule
-    int N, G, Gp, K, B;
-    uint32_t seed, key;            // Philox key words: (seed, chain id)
-    int n_burnin, n_unchanged, n_pres, traces;
-    int TR, T_max;                 // rows per tile, tile capacity
-    int prec;                      // 32 or 64
+// Fields of one chain.  Q qualifies the pointee of every device address:
+// empty on the host, the global address space in kernels (so that loads,
+// stores and atomics through them are global, not flat, instructions).
+#define BQ_CHAIN_FIELDS(Q) \
+    /* dimensions and schedule */ \
+    int N, G, Gp, K, B; \
+    uint32_t seed, key;  /* Philox key words: (seed, chain id) */ \
+    int n_burnin, n_unchanged, n_pres, traces; \
+    int TR, T_max;  /* rows per tile, tile capacity */ \
+    int prec;  /* 32 or 64 */ \
+    /* static data */ \
+    const Q int32_t* batch;  /* [N] batch of each cell */ \
+    const Q int32_t* boff;  /* [B+1] first cell of each batch */ \
+    const Q int64_t* hist_s;  /* [B*H] static histogram of Y>0 entries with Y<H */ \
+    const Q int32_t* zrow;  /* [N+1] CSR row offsets of the Y = 0 entries */ \
+    const Q uint16_t* zcol;  /* [nz]  their genes */ \
+    const Q int32_t* big_off;  /* [B+1] static counts Y >= H, per batch */ \
+    const Q int32_t* big;  /* [nbig] their values */ \
+    /* state */ \
+    Q int* iter;  /* [1] global iteration (sweep) index */ \
+    Q int32_t* S;  /* [N*Gp] encoded X/Z (see numerics contract) */ \
+    Q int32_t* W;  /* [N] */ \
+    Q int32_t* L;  /* [K*G] */ \
+    Q double *alpha, *beta, *nu, *delta, *phi, *gamma, *pi, *ptau; \
+    Q double *mu_alpha, *mu_nu, *mu_delta; \
+    /* precision-T copies of the parameters */ \
+    Q void *ea, *eb, *em, *bt, *en, *ed, *ph; \
+    /* proposals and their precision-T constants */ \
+    Q double *alpha_p, *beta_p, *nu_p, *phi_p, *gamma_p; \
+    Q void *da, *ca, *db, *cb, *dn, *cn; \
+    Q void* pc;  /* [6][B*G]: phi', d, d/phi, 1/phi', log(phi'/phi), lgamma diff */ \
+    /* plan of the sweep */ \
+    Q int32_t* cnt;  /* [BQ_STRIPES][B*K] cells per (batch, subtype), striped */ \
+    Q int32_t* perm;  /* [N] cells sorted stably by (batch, subtype, index) */ \
+    Q int32_t* tiles;  /* [T_max*4] (b, k, start, count) */ \
+    Q int32_t* tile_off;  /* [B*K+1] first tile of each (batch, subtype) */ \
+    Q int32_t* ntiles;  /* [1] */ \
+    Q uint64_t* part;  /* [T_max][3][Gp] exact tile partial sums */ \
+    /* gamma statistics */ \
+    Q uint32_t* hist_d;  /* [B*H] dynamic histogram (entries with Y = 0), values >= BQ_HLO */ \
+    Q uint32_t* hist_lo;  /* [BQ_STRIPES][B*BQ_HLO] the same for values < BQ_HLO, striped */ \
+    Q uint64_t* gstat;  /* [BQ_STRIPES][B*5] n_z1, sum x z, over0, over1_rej, over1_acc, striped */ \
+    Q double* sp_tab;  /* [B*H] log(1 + e^{-(gamma0 + gamma1 x)}) for the current gamma */ \
+    Q int32_t* xz_tail;  /* [nz] per cell, at the cell's zero-entry offset: entries left to the gamma-Poisson draw */ \
+    Q int32_t* xz_ntail;  /* [N] their number per cell this sweep */ \
+    Q int32_t* beta_acc;  /* [K*G] 1 where this sweep accepted beta_kg */ \
+    /* posterior accumulators */ \
+    Q int* n_rec; \
+    Q double *s_gamma, *s_alpha, *s_beta, *s_nu, *s_delta, *s_phi, *s_pi, *s_ptau; \
+    Q int32_t *c_L, *c_W; \
+    /* traces [param * n_pres + r] */ \
+    Q double *tr_gamma, *tr_alpha, *tr_beta, *tr_ptau, *tr_nu, *tr_delta, *tr_phi, *tr_pi; \
+    Q int32_t *tr_L, *tr_W; \
+    /* end */
 
-    // static data
-    const int32_t* batch;          // [N] batch of each cell
-    const int32_t* boff;           // [B+1] first cell of each batch
-    const int64_t* hist_s;         // [B*H] static histogram of Y>0 entries with Y<H
-    const int32_t* zrow;           // [N+1] CSR row offsets of the Y = 0 entries
-    const uint16_t* zcol;          // [nz]  their genes
-    const int32_t* big_off;        // [B+1] static counts Y >= H, per batch
-    const int32_t* big;            // [nbig] their values
+struct ChainDev { BQ_CHAIN_FIELDS() };
 
-    // state
-    int* iter;                     // [1] global iteration (sweep) index
-    int32_t* S;                    // [N*Gp] encoded X/Z (see numerics contract)
-    int32_t* W;                    // [N]
-    int32_t* L;                    // [K*G]
-    double *alpha, *beta, *nu, *delta, *phi, *gamma, *pi, *ptau;
-    double *mu_alpha, *mu_nu, *mu_delta;
-
-    // precision-T copies of the parameters
-    void *ea, *eb, *em, *bt, *en, *ed, *ph;
-
-    // proposals and their precision-T constants
-    double *alpha_p, *beta_p, *nu_p, *phi_p, *gamma_p;
-    void *da, *ca, *db, *cb, *dn, *cn;
-    void* pc;                      // [6][B*G]: phi', d, d/phi, 1/phi', log(phi'/phi), lgamma diff
-
-    // plan of the sweep
-    int32_t* cnt;                  // [B*K] cells per (batch, subtype)
-    int32_t* perm;                 // [N] cells sorted stably by (batch, subtype, index)
-    int32_t* tiles;                // [T_max*4] (b, k, start, count)
-    int32_t* tile_off;             // [B*K+1] first tile of each (batch, subtype)
-    int32_t* ntiles;               // [1]
-    uint64_t* part;                // [T_max][3][Gp] exact tile partial sums
-
-    // gamma statistics
-    uint32_t* hist_d;              // [B*H] dynamic histogram (entries with Y = 0)
-    uint64_t* gstat;               // [B*5] n_z1, sum x z, over0, over1_rej, over1_acc
-    double* sp_tab;                // [B*H] log(1 + e^{-(gamma0 + gamma1 x)}) for the current gamma
-
-    int32_t* beta_acc;             // [K*G] 1 where this sweep accepted beta_kg
-
-    // posterior accumulators
-    int* n_rec;
-    double *s_gamma, *s_alpha, *s_beta, *s_nu, *s_delta, *s_phi, *s_pi, *s_ptau;
-    int32_t *c_L, *c_W;
-    // traces [param * n_pres + r]
-    double *tr_gamma, *tr_alpha, *tr_beta, *tr_ptau, *tr_nu, *tr_delta, *tr_phi, *tr_pi;
-    int32_t *tr_L, *tr_W;
-};
+#ifdef __HIPCC__
+#define BQ_GLOBAL __attribute__((address_space(1)))
+// kernel view of ChainDev: the same layout, pointers into the global address space
+struct ChainG { BQ_CHAIN_FIELDS(BQ_GLOBAL) };
+static_assert(sizeof(ChainG) == sizeof(ChainDev), "ChainG mirrors ChainDev");
+#endif
 
 // Host-side initial state of one chain (BUSseq_gpu.cu:1163-1244).
 struct HostInit {

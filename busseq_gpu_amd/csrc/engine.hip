@@ -88,7 +88,12 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh)
     return r;
 }
 
-__device__ __forceinline__ double ph_uniform(const ChainDev& c, uint32_t a, uint32_t b, int t, int phase)
+// atomics on global-space addresses (the cast back to a generic pointer is
+// undone by the compiler's address-space inference: the instruction is global)
+template <typename V>
+__device__ __forceinline__ V gatomic_add(BQ_GLOBAL V* p, V v) { return atomicAdd((V*)p, v); }
+
+__device__ __forceinline__ double ph_uniform(const ChainG& c, uint32_t a, uint32_t b, int t, int phase)
 {
     bq_rng r;
     bq_rng_init(&r, c.seed, c.key, a, b, (uint32_t)t, (uint32_t)phase);
@@ -102,7 +107,7 @@ __device__ __forceinline__ double ph_uniform(const ChainDev& c, uint32_t a, uint
 template <typename T>
 __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int t = *c.iter;
     if (j < c.G) {
@@ -123,14 +128,16 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
         bq_rng r1;
         bq_rng_init(&r1, c.seed, c.key, (uint32_t)j, 0u, (uint32_t)t, BQ_PH_GAMMA1_P);
         c.gamma_p[B + j] = -bq_rgamma(&r1, -10.0 * c.gamma[B + j], 0.1);
-        for (int q = 0; q < 5; ++q) c.gstat[j * 5 + q] = 0;
+        for (int r = 0; r < BQ_STRIPES; ++r)
+            for (int q = 0; q < 5; ++q) c.gstat[(size_t)r * B * 5 + j * 5 + q] = 0;
     }
+    if (j < c.B * BQ_HLO * BQ_STRIPES) c.hist_lo[j] = 0u;
     if (j < c.B * H) {
         c.hist_d[j] = 0u;
         const int b = j / H, x = j - b * H;
         c.sp_tab[j] = bq_neg_logit_sp(c.gamma[b], c.gamma[c.B + b], (double)x);
     }
-    if (j < c.B * c.K) c.cnt[j] = 0;
+    if (j < c.B * c.K * BQ_STRIPES) c.cnt[j] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -141,7 +148,7 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
 template <typename T, int KT>
 __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= c.N) return;
@@ -153,7 +160,7 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
     const T* bt = (const T*)c.bt;
     const T* em = (const T*)c.em;
     const T edi = ((const T*)c.ed)[i];
-    const int32_t* row = c.S + (size_t)i * c.Gp;
+    auto row = c.S + (size_t)i * c.Gp;
 
     uint64_t acc[KT];
 #pragma unroll
@@ -196,7 +203,7 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
         if (!found) { cum += lp[k]; if (tgt <= cum) { kk = k; found = 1; } }
     if (lane == 0) {
         c.W[i] = kk;
-        atomicAdd(&c.cnt[b * KT + kk], 1);
+        gatomic_add(&c.cnt[(size_t)(blockIdx.x % BQ_STRIPES) * c.B * KT + b * KT + kk], 1);
     }
 }
 
@@ -219,7 +226,7 @@ void launch_w(int K, dim3 grid, dim3 blk, hipStream_t st, const ChainDev* cd)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int b = blockIdx.x;
     if (b >= c.B) return;
     const int K = c.K, B = c.B, TR = c.TR, tid = threadIdx.x;
@@ -227,7 +234,11 @@ __global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
     __shared__ int s_toff[BQ_BMAX * BQ_KMAX + 1];
     __shared__ int s_cnt[BQ_BMAX * BQ_KMAX];
     __shared__ int s_pos[256 * BQ_KMAX];
-    for (int j = tid; j < B * K; j += 256) s_cnt[j] = c.cnt[j];
+    for (int j = tid; j < B * K; j += 256) {
+        int n = 0;
+        for (int r = 0; r < BQ_STRIPES; ++r) n += c.cnt[(size_t)r * B * K + j];
+        s_cnt[j] = n;
+    }
     __syncthreads();
     if (tid == 0) {
         int acc = 0;
@@ -299,15 +310,21 @@ __global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
 // lanes over the cell's static list of zero entries; dynamic gamma histogram.
 // The Z draw runs on every entry; entries with Z = 1 are queued in LDS with
 // their random-stream state and their negative-binomial proposals are drawn
-// 64 at a time, so the long double-precision path runs with full lanes.
-// An entry is written back only when its state changes.
+// 64 at a time, so the proposal path runs with full lanes.  Proposals with
+// a small mean are drawn here by CDF inversion; the others (mean > 30 or a
+// heavy tail) and entries whose current X is >= H are listed, in the cell's
+// own segment of the tail list, for k_xz_tail, which redoes the whole update
+// of such an entry with the gamma-Poisson draw.  Every value k_xz writes is
+// below H.
+// The update of every entry is bq_xz_update; an entry is written back only
+// when its state changes.
 // ---------------------------------------------------------------------------
 struct XzItem {
+    int p;                 // index of the entry in the zero-entry list
     int g;
     int32_t s;
+    uint32_t b0, b1, b2;   // unread words of the entry's Philox block after the Z draw
     double mu;
-    uint32_t b0, b1, b2;
-    int n;
 };
 
 __device__ __forceinline__ void wave_lds_sync()
@@ -316,34 +333,50 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
+// statistics of one updated entry with Z = 1 and value xi >= H for the gamma
+// sums (the entries below H go to the histogram)
+__device__ __forceinline__ void xz_big_terms(double g0, double g1, double g0p, double g1p, int xi,
+                                             uint64_t& o0, uint64_t& o1r, uint64_t& o1a)
+{
+    const double xd = (double)xi;
+    const double a0 = bq_sp_lin(g0, g1, xd), a1 = bq_sp_lin(g0p, g1, xd);
+    const double a2 = bq_sp_lin(g0, g1p, xd), a3 = bq_sp_lin(g0p, g1p, xd);
+    o0 += bq_fix(a0 - a1);
+    o1r += bq_fix(a0 - a2);
+    o1a += bq_fix(a1 - a3);
+}
+
+static_assert(BQ_HLO == 64, "k_xz keeps one low histogram bin per lane");
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
-    __shared__ uint32_t hist[4][H];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    __shared__ uint32_t hist[4][BQ_HLO];
     __shared__ XzItem queue[4][128];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 4 + wv;
     if (i >= c.N) return;
     uint32_t* hw = hist[wv];
     XzItem* qw = queue[wv];
-    for (int x = lane; x < H; x += 64) hw[x] = 0u;
-    const int G = c.G, B = c.B, t = *c.iter;
+    hw[lane] = 0u;
+    const int G = c.G, t = *c.iter;
     const int b = c.batch[i], k = c.W[i];
-    const double g0 = c.gamma[b], g1 = c.gamma[B + b];
-    const double g0p = c.gamma_p[b], g1p = c.gamma_p[B + b];
+    const double g0 = c.gamma[b];
     const double qz = 1.0 / (1.0 + bq_exp(g0));
     const T edi = ((const T*)c.ed)[i];
     const T* ea = (const T*)c.ea;
     const T* eb = (const T*)c.eb + (size_t)k * G;
     const T* en = (const T*)c.en + (size_t)b * G;
-    const double* phi = c.phi + (size_t)b * G;
-    const double* spt = c.sp_tab + (size_t)b * H;
-    int32_t* row = c.S + (size_t)i * c.Gp;
+    auto phi = c.phi + (size_t)b * G;
+    auto spt = c.sp_tab + (size_t)b * H;
+    auto row = c.S + (size_t)i * c.Gp;
     const int z0 = c.zrow[i], z1 = c.zrow[i + 1];
     const uint64_t below = (1ull << lane) - 1ull;
-    uint64_t n1 = 0, sx1 = 0, o0 = 0, o1r = 0, o1a = 0;
-    int qn = 0;
+    uint64_t n1 = 0, sx1 = 0;
+    int qn = 0, nt = 0;
+    const int stripe = blockIdx.x % BQ_STRIPES;
+    auto tl = c.xz_tail + z0;
     for (int base = z0;; base += 64) {
         const bool more = base < z1;
         if (more) {
@@ -360,50 +393,55 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
                 z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
             }
             const uint64_t m = __ballot(act && z);
+            const uint64_t m0 = __ballot(act && !z);
             if (act && z) {
                 XzItem it;
+                it.p = p;
                 it.g = g;
                 it.s = s;
+                it.b0 = r.b0; it.b1 = r.b1; it.b2 = r.b2;
                 it.mu = (double)tmu(ea[g], eb[g], en[g], edi);
-                it.b0 = r.b0; it.b1 = r.b1; it.b2 = r.b2; it.n = r.n;
                 qw[qn + __popcll(m & below)] = it;
             }
-            if (act && !z) {
-                if (s != 0) row[g] = 0;
-                atomicAdd(&hw[0], 1u);
-            }
+            if (act && !z && s != 0) row[g] = 0;
+            if (lane == 0 && m0) atomicAdd(&hw[0], (uint32_t)__popcll(m0));
             qn += __popcll(m);
         }
         if (qn >= 64 || (!more && qn > 0)) {
             const int take = qn < 64 ? qn : 64;
             wave_lds_sync();
+            bool tail = false;
+            int tp = 0;
             if (lane < take) {
                 const XzItem it = qw[qn - take + lane];
                 const int g = it.g;
                 const int xo = it.s < 0 ? ~it.s : 0;
+                // the stream as bq_xz_update leaves it after the Z draw (three
+                // buffered words) or, without a Z draw, before its first block
                 bq_rng r;
                 r.k0 = c.seed; r.k1 = c.key; r.c0 = (uint32_t)i; r.c1 = (uint32_t)g; r.c2 = (uint32_t)t;
-                r.c3 = ((uint32_t)BQ_PH_XZ << 20) + (it.n > 0 ? 1u : 0u);
-                r.b0 = it.b0; r.b1 = it.b1; r.b2 = it.b2; r.b3 = 0u; r.n = it.n;
-                int xp = bq_rnbinom(&r, it.mu, phi[g]);
-                double lo = xo < H ? spt[xo] : bq_neg_logit_sp(g0, g1, (double)xo);
-                double lp = xp < H ? spt[xp] : bq_neg_logit_sp(g0, g1, (double)xp);
-                int xi = bq_log(bq_uniform(&r)) <= lo - lp ? xp : xo;
-                const int32_t sn = ~xi;
-                if (sn != it.s) row[g] = sn;
-                n1 += 1;
-                sx1 += (uint64_t)xi;
-                if (xi < H) {
-                    atomicAdd(&hw[xi], 1u);
+                r.c3 = ((uint32_t)BQ_PH_XZ << 20) + (xo == 0 ? 1u : 0u);
+                r.b0 = it.b0; r.b1 = it.b1; r.b2 = it.b2; r.b3 = 0u; r.n = xo == 0 ? 3 : 0;
+                const double u = bq_uniform(&r);
+                const double ph = phi[g];
+                if (xo < H && bq_nb_inv_ok(it.mu, ph)) {
+                    // the inversion returns x' <= 255 < H: both softplus terms are table entries
+                    const int xp = bq_rnbinom_inv(u, it.mu, ph);
+                    const int xi = bq_log(bq_uniform(&r)) <= spt[xo] - spt[xp] ? xp : xo;
+                    const int32_t sn = ~xi;
+                    if (sn != it.s) row[g] = sn;
+                    n1 += 1;
+                    sx1 += (uint64_t)xi;
+                    if (xi < BQ_HLO) atomicAdd(&hw[xi], 1u);
+                    else gatomic_add(&c.hist_d[b * H + xi], 1u);
                 } else {
-                    double xd = (double)xi;
-                    double a0 = bq_sp_lin(g0, g1, xd), a1 = bq_sp_lin(g0p, g1, xd);
-                    double a2 = bq_sp_lin(g0, g1p, xd), a3 = bq_sp_lin(g0p, g1p, xd);
-                    o0 += bq_fix(a0 - a1);
-                    o1r += bq_fix(a0 - a2);
-                    o1a += bq_fix(a1 - a3);
+                    tail = true;
+                    tp = it.p;
                 }
             }
+            const uint64_t tm = __ballot(tail);
+            if (tail) tl[nt + __popcll(tm & below)] = tp;
+            nt += __popcll(tm);
             qn -= take;
             wave_lds_sync();
         }
@@ -411,24 +449,96 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
     }
     n1 = wave_sum_u64(n1);
     sx1 = wave_sum_u64(sx1);
-    o0 = wave_sum_u64(o0);
-    o1r = wave_sum_u64(o1r);
-    o1a = wave_sum_u64(o1a);
     wave_lds_sync();
-    for (int x = lane; x < H; x += 64) {
-        uint32_t v = hw[x];
-        if (v) atomicAdd(&c.hist_d[b * H + x], v);
+    {
+        const uint32_t v = hw[lane];
+        if (v) gatomic_add(&c.hist_lo[(size_t)stripe * c.B * BQ_HLO + b * BQ_HLO + lane], v);
     }
     if (lane == 0) {
-        unsigned long long* gs = (unsigned long long*)(c.gstat + b * 5);
+        c.xz_ntail[i] = nt;
+        unsigned long long* gs = (unsigned long long*)(c.gstat + (size_t)stripe * c.B * 5 + b * 5);
         if (n1) atomicAdd(gs + 0, (unsigned long long)n1);
         if (sx1) atomicAdd(gs + 1, (unsigned long long)sx1);
-        if (o0 || o1r || o1a) {
-            atomicAdd(gs + 2, (unsigned long long)o0);
-            atomicAdd(gs + 3, (unsigned long long)o1r);
-            atomicAdd(gs + 4, (unsigned long long)o1a);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_xz_tail: the entries k_xz left to the gamma-Poisson proposal.  A wave
+// takes TAIL_CPW consecutive cells and runs over the concatenation of their
+// listed entries with full lanes; the complete update bq_xz_update runs from
+// each entry's stream.  Counts and gamma sums are gathered per batch in LDS
+// and added once per block.
+// ---------------------------------------------------------------------------
+#define TAIL_CPW 8
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_xz_tail(const ChainDev* __restrict__ cds)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    __shared__ unsigned long long st[BQ_BMAX * 5];
+    __shared__ uint32_t hl[BQ_BMAX * BQ_HLO];
+    __shared__ int incl[4][TAIL_CPW];
+    const int N = c.N, G = c.G, B = c.B, t = *c.iter;
+    const int cb0 = blockIdx.x * 4 * TAIL_CPW;
+    if (cb0 >= N) return;
+    const int cb1 = (cb0 + 4 * TAIL_CPW < N ? cb0 + 4 * TAIL_CPW : N) - 1;
+    const int bmin = c.batch[cb0], nbb = c.batch[cb1] - bmin + 1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i0 = cb0 + wv * TAIL_CPW;
+    const int stripe = blockIdx.x % BQ_STRIPES;
+    for (int q = threadIdx.x; q < nbb * 5; q += 256) st[q] = 0ull;
+    for (int q = threadIdx.x; q < nbb * BQ_HLO; q += 256) hl[q] = 0u;
+    // inclusive prefix of the listed counts over the wave's cells
+    int v = (lane < TAIL_CPW && i0 + lane < N) ? c.xz_ntail[i0 + lane] : 0;
+#pragma unroll
+    for (int o = 1; o < TAIL_CPW; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    if (lane < TAIL_CPW) incl[wv][lane] = v;
+    __syncthreads();
+    const int total = incl[wv][TAIL_CPW - 1];
+    for (int j = lane; j < total; j += 64) {
+        int m = 0;
+#pragma unroll
+        for (int q = 0; q < TAIL_CPW - 1; ++q) m += incl[wv][q] <= j;
+        const int i = i0 + m;
+        const int l = j - (m ? incl[wv][m - 1] : 0);
+        const int g = c.zcol[c.xz_tail[c.zrow[i] + l]];
+        const int b = c.batch[i], k = c.W[i];
+        auto e = c.S + (size_t)i * c.Gp + g;
+        const int32_t s = *e;
+        const double g0 = c.gamma[b], g1 = c.gamma[B + b];
+        const double qz = 1.0 / (1.0 + bq_exp(g0));
+        const T mu = tmu(((const T*)c.ea)[g], ((const T*)c.eb)[(size_t)k * G + g],
+                         ((const T*)c.en)[(size_t)b * G + g], ((const T*)c.ed)[i]);
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ);
+        const int32_t sn = bq_xz_update(s, (double)mu, c.phi[(size_t)b * G + g], g0, g1, qz, &r);
+        if (sn != s) *e = sn;
+        const int xi = sn < 0 ? ~sn : 0;
+        const int bl = b - bmin;
+        if (sn < 0) {
+            atomicAdd(&st[bl * 5 + 0], 1ull);
+            atomicAdd(&st[bl * 5 + 1], (unsigned long long)xi);
+        }
+        if (xi < BQ_HLO) {
+            atomicAdd(&hl[bl * BQ_HLO + xi], 1u);
+        } else if (xi < H) {
+            gatomic_add(&c.hist_d[b * H + xi], 1u);
+        } else {
+            uint64_t o0 = 0, o1r = 0, o1a = 0;
+            xz_big_terms(g0, g1, c.gamma_p[b], c.gamma_p[B + b], xi, o0, o1r, o1a);
+            atomicAdd(&st[bl * 5 + 2], (unsigned long long)o0);
+            atomicAdd(&st[bl * 5 + 3], (unsigned long long)o1r);
+            atomicAdd(&st[bl * 5 + 4], (unsigned long long)o1a);
         }
     }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nbb * 5; q += 256)
+        if (st[q]) atomicAdd((unsigned long long*)c.gstat + (size_t)stripe * B * 5 + bmin * 5 + q, st[q]);
+    for (int q = threadIdx.x; q < nbb * BQ_HLO; q += 256)
+        if (hl[q]) gatomic_add(&c.hist_lo[(size_t)stripe * B * BQ_HLO + bmin * BQ_HLO + q], hl[q]);
 }
 
 // ---------------------------------------------------------------------------
@@ -437,7 +547,7 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
 template <typename T>
 __global__ __launch_bounds__(256) void k_alpha(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int tile = blockIdx.y;
     if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -472,14 +582,14 @@ __global__ __launch_bounds__(256) void k_alpha(const ChainDev* __restrict__ cds)
 template <typename T>
 __global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int G = c.G, K = c.K, B = c.B;
     int j = blockIdx.x * 256 + threadIdx.x;
     const int t = *c.iter;
     if (j < G) {
         const int g = j, nt = *c.ntiles;
         const size_t stride = (size_t)3 * c.Gp;
-        const uint64_t* pp = c.part + g;
+        auto pp = c.part + g;
         uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
         int tt = 0;
         for (; tt + 4 <= nt; tt += 4) {
@@ -555,10 +665,12 @@ __global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ uint64_t s_red[4];
     __shared__ double s_g0;
     __shared__ int s_acc0;
+    __shared__ int64_t s_n[H];        // histogram of X over the batch's entries (static + dynamic)
+    __shared__ uint64_t s_gs[5];      // the batch's gamma statistics, stripes added
     const int tid = threadIdx.x, B = c.B, t = *c.iter;
     if (blockIdx.x == gridDim.x - 1) {
         if (t < c.n_unchanged) return;
@@ -584,13 +696,27 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
     }
     const int b = blockIdx.x;
     if (b >= B) return;
-    const int64_t* hs = c.hist_s + (size_t)b * H;
-    const uint32_t* hd = c.hist_d + (size_t)b * H;
+    auto hs = c.hist_s + (size_t)b * H;
+    for (int x = tid; x < H; x += 256) {
+        int64_t n = hs[x];
+        if (x < BQ_HLO) {
+            for (int r = 0; r < BQ_STRIPES; ++r) n += (int64_t)c.hist_lo[(size_t)r * B * BQ_HLO + b * BQ_HLO + x];
+        } else {
+            n += (int64_t)c.hist_d[(size_t)b * H + x];
+        }
+        s_n[x] = n;
+    }
+    if (tid < 5) {
+        uint64_t v = 0;
+        for (int r = 0; r < BQ_STRIPES; ++r) v += c.gstat[(size_t)r * B * 5 + b * 5 + tid];
+        s_gs[tid] = v;
+    }
+    __syncthreads();
     const double g0 = c.gamma[b], g1 = c.gamma[B + b];
     const double g0p = c.gamma_p[b], g1p = c.gamma_p[B + b];
     uint64_t a0 = 0;
     for (int x = tid; x < H; x += 256) {
-        int64_t n = hs[x] + (int64_t)hd[x];
+        int64_t n = s_n[x];
         if (n) a0 += bq_fix((double)n * (bq_sp_lin(g0, g1, (double)x) - bq_sp_lin(g0p, g1, (double)x)));
     }
     const int bb0 = c.big_off[b], bb1 = c.big_off[b + 1];
@@ -600,7 +726,7 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
     }
     a0 = block_sum_u64(a0, s_red);
     if (tid == 0) {
-        const uint64_t* gs = c.gstat + b * 5;
+        const uint64_t* gs = s_gs;
         a0 += bq_fix((g0p - g0) * (double)(int64_t)gs[0]);
         a0 += gs[2];
         double s0 = bq_unfix(a0);
@@ -614,7 +740,7 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
     const double g0n = s_g0;
     uint64_t a1 = 0;
     for (int x = tid; x < H; x += 256) {
-        int64_t n = hs[x] + (int64_t)hd[x];
+        int64_t n = s_n[x];
         if (n) a1 += bq_fix((double)n * (bq_sp_lin(g0n, g1, (double)x) - bq_sp_lin(g0n, g1p, (double)x)));
     }
     for (int q = bb0 + tid; q < bb1; q += 256) {
@@ -623,7 +749,7 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
     }
     a1 = block_sum_u64(a1, s_red);
     if (tid == 0) {
-        const uint64_t* gs = c.gstat + b * 5;
+        const uint64_t* gs = s_gs;
         a1 += bq_fix((g1p - g1) * (double)(int64_t)gs[1]);
         a1 += s_acc0 ? gs[4] : gs[3];
         double s1 = bq_unfix(a1);
@@ -642,7 +768,7 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
 template <typename T>
 __global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int tile = blockIdx.y;
     if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -675,7 +801,7 @@ __global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cd
             if (k >= 1) pn1 += (double)ttmh2(x, phg, rt, dng, cbn, lb);
         }
     }
-    uint64_t* pt = c.part + (size_t)tile * 3 * c.Gp;
+    auto pt = c.part + (size_t)tile * 3 * c.Gp;
     pt[g] = bq_fix(pb);
     pt[c.Gp + g] = bq_fix(pn0);
     pt[2 * (size_t)c.Gp + g] = bq_fix(pn1);
@@ -687,13 +813,13 @@ __global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cd
 template <typename T>
 __global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int G = c.G, K = c.K, B = c.B;
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= G * (K - 1)) return;
     const int k = 1 + j / G, g = j % G, t = *c.iter;
     const size_t Gp = c.Gp;
-    const int* toff = c.tile_off;
+    auto toff = c.tile_off;
     uint64_t a = 0;
     for (int b = 0; b < B; ++b) {
         int e = toff[b * K + k + 1];
@@ -720,13 +846,13 @@ __global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
 template <typename T>
 __global__ __launch_bounds__(256) void k_n_dec(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int G = c.G, K = c.K, B = c.B;
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= G * (B - 1)) return;
     const int b = 1 + j / G, g = j % G, t = *c.iter;
     const size_t Gp = c.Gp;
-    const int* toff = c.tile_off;
+    auto toff = c.tile_off;
     uint64_t a = 0;
     for (int k = 0; k < K; ++k) {
         const int slot = (k >= 1 && c.beta_acc[(size_t)k * G + g]) ? 2 : 1;
@@ -750,7 +876,7 @@ __global__ __launch_bounds__(256) void k_n_dec(const ChainDev* __restrict__ cds)
 template <typename T>
 __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= c.N) return;
@@ -770,7 +896,7 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
     const double dp = del + BQ_RW_SD * bq_normal(&r);
     const double dl = dp - del;
     const T dd = (T)dl, cd = (T)bq_expm1(dl);
-    const int32_t* row = c.S + (size_t)i * c.Gp;
+    auto row = c.S + (size_t)i * c.Gp;
     uint64_t a = 0;
     for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
         const int c1 = c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G;
@@ -802,7 +928,7 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
 template <typename T>
 __global__ __launch_bounds__(256) void k_phi(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int tile = blockIdx.y;
     if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -834,7 +960,7 @@ __global__ __launch_bounds__(256) void k_phi(const ChainDev* __restrict__ cds)
 template <typename T>
 __global__ __launch_bounds__(256) void k_phi_dec(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int G = c.G, K = c.K, B = c.B;
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= G * B) return;
@@ -860,7 +986,7 @@ __global__ __launch_bounds__(256) void k_phi_dec(const ChainDev* __restrict__ cd
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_record(const ChainDev* __restrict__ cds)
 {
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int t = *c.iter;
     if (t < c.n_burnin) return;
     const int r = t - c.n_burnin;
@@ -922,7 +1048,7 @@ __global__ __launch_bounds__(256) void k_bic(const ChainDev* __restrict__ cds, c
                                              double* __restrict__ ll)
 {
     // pm: posterior means packed as alpha[G] | beta[K*G] | nu[B*G] | delta[N] | phi[B*G] | gamma[2B]
-    const ChainDev& c = cds[blockIdx.z];
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int G = c.G, K = c.K, B = c.B, N = c.N;
@@ -934,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_bic(const ChainDev* __restrict__ cds, c
     const double* de = nu + (size_t)B * G;
     const double* phi = de + N;
     const double* ga = phi + (size_t)B * G;
-    const int32_t* Srow = c.S + (size_t)i * c.Gp;
+    auto Srow = c.S + (size_t)i * c.Gp;
     uint64_t a = 0;
     for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
         const int c1 = c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G;
@@ -982,15 +1108,16 @@ struct busseq_sampler {
     hipGraphExec_t gexec = nullptr;
     std::vector<void*> allocs;
     int iter = 0;
+    bool oom = false;              // some dalloc failed
 
     template <typename V>
     V* dalloc(size_t n)
     {
         void* p = nullptr;
         if (n == 0) n = 1;
-        if (hipMalloc(&p, n * sizeof(V)) != hipSuccess) return nullptr;
+        if (hipMalloc(&p, n * sizeof(V)) != hipSuccess) { oom = true; return nullptr; }
         allocs.push_back(p);
-        if (hipMemset(p, 0, n * sizeof(V)) != hipSuccess) return nullptr;
+        if (hipMemset(p, 0, n * sizeof(V)) != hipSuccess) { oom = true; return nullptr; }
         return (V*)p;
     }
 };
@@ -998,9 +1125,9 @@ struct busseq_sampler {
 namespace {
 
 static const char* const kKernelNames[] = {
-    "k_prep", "k_w", "k_plan", "k_xz", "k_alpha", "k_alpha_dec", "k_gamma_ptau", "k_beta_nu",
-    "k_b_dec", "k_n_dec", "k_delta", "k_phi", "k_phi_dec", "k_record", "k_tick"};
-static const int kNumKernels = 15;
+    "k_prep", "k_w", "k_plan", "k_xz", "k_xz_tail", "k_alpha", "k_alpha_dec", "k_gamma_ptau",
+    "k_beta_nu", "k_b_dec", "k_n_dec", "k_delta", "k_phi", "k_phi_dec", "k_record", "k_tick"};
+static const int kNumKernels = 16;
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
@@ -1030,6 +1157,7 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     }
     EV(); hipLaunchKernelGGL(k_plan, g_plan, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_xz<T>, g_rows, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_xz_tail<T>, dim3((s->maxN + 4 * TAIL_CPW - 1) / (4 * TAIL_CPW), 1, C), blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha<T>, g_tile, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_items, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, st, cd);
@@ -1192,6 +1320,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             HostInit hi;
             uint32_t key = (uint32_t)(cfg->chain0 + k);
             if (bq_host_init(B, n_b.data(), G, Gp, K, Y, cfg->seed, key, hi) != 0) return fail(BUSSEQ_ERR_DATA);
+            const long nz = (long)hi.zcol.size();   // entries with Y = 0
             if (k == 0) {
                 pd.batch = s->dalloc<int32_t>(N); pd.boff = s->dalloc<int32_t>(B + 1);
                 pd.hist = s->dalloc<int64_t>((size_t)B * H); pd.zrow = s->dalloc<int32_t>((size_t)N + 1);
@@ -1236,14 +1365,17 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.db = s->dalloc<char>(ts * K * G); c.cb = s->dalloc<char>(ts * K * G);
             c.dn = s->dalloc<char>(ts * B * G); c.cn = s->dalloc<char>(ts * B * G);
             c.pc = s->dalloc<char>(ts * 6 * B * G);
-            c.cnt = s->dalloc<int32_t>((size_t)B * K);
+            c.cnt = s->dalloc<int32_t>((size_t)B * K * BQ_STRIPES);
             c.perm = s->dalloc<int32_t>(N);
             c.tiles = s->dalloc<int32_t>((size_t)T_max * 4);
             c.tile_off = s->dalloc<int32_t>((size_t)B * K + 1);
             c.ntiles = s->dalloc<int32_t>(1);
             c.part = s->dalloc<uint64_t>((size_t)T_max * 3 * Gp);
             c.hist_d = s->dalloc<uint32_t>((size_t)B * H);
-            c.gstat = s->dalloc<uint64_t>((size_t)B * 5);
+            c.hist_lo = s->dalloc<uint32_t>((size_t)B * BQ_HLO * BQ_STRIPES);
+            c.gstat = s->dalloc<uint64_t>((size_t)B * 5 * BQ_STRIPES);
+            c.xz_tail = s->dalloc<int32_t>(nz > 0 ? (size_t)nz : 1);
+            c.xz_ntail = s->dalloc<int32_t>(N);
             c.sp_tab = s->dalloc<double>((size_t)B * H);
             c.beta_acc = s->dalloc<int32_t>((size_t)K * G);
             c.n_rec = s->dalloc<int>(1);
@@ -1260,7 +1392,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
                 c.tr_L = s->dalloc<int32_t>((size_t)(K - 1) * G * P); c.tr_W = s->dalloc<int32_t>((size_t)N * P);
                 if (!c.tr_W || !c.tr_L || !c.tr_delta) { bq_set_error("device allocation of traces failed"); return fail(BUSSEQ_ERR_NOMEM); }
             }
-            if (!c.part || !c.S || !c.c_W || !c.pc || !c.ptau || !c.sp_tab) {
+            if (s->oom) {
                 bq_set_error("device allocation failed");
                 return fail(BUSSEQ_ERR_NOMEM);
             }
@@ -1280,7 +1412,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             if (rc != BUSSEQ_OK) return fail(rc);
             auto mx = [](int& a, long v) { if (v > a) a = (int)v; };
             mx(s->maxN, N); mx(s->maxG, G); mx(s->maxK, K); mx(s->maxB, B); mx(s->maxT, T_max);
-            mx(s->maxPrep, G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * K);
+            mx(s->maxPrep, G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * BQ_HLO * BQ_STRIPES); mx(s->maxPrep, (long)B * K * BQ_STRIPES);
             mx(s->maxItems, (long)G * (1 + (K - 1) + (B - 1) + B));
             mx(s->maxRec, N); mx(s->maxRec, (long)K * G); mx(s->maxRec, (long)B * G); mx(s->maxRec, (long)B * K);
             mx(s->maxRec, 2L * B);

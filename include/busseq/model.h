@@ -159,6 +159,24 @@ BQ_FN double bq_neg_logit_sp(double g0, double g1, double x)
     return bq_softplus(-(g0 + g1 * x));
 }
 
+/*
+ * One X/Z update of an entry with Y = 0 and encoded state s (cu:484-512): Z is
+ * redrawn when X = 0 (P(Z = 1) = 1 - qz, qz = 1/(1 + e^{g0})); with Z = 1 a
+ * proposal x' ~ NB(mu, phi) is accepted with probability
+ * min(1, (1 + e^{-g0-g1 X}) / (1 + e^{-g0-g1 x'})), compared in log space.
+ * Returns the new encoded state (0, or ~X with Z = 1).
+ */
+BQ_FN int32_t bq_xz_update(int32_t s, double mu, double phi, double g0, double g1, double qz, bq_rng* r)
+{
+    int xo = s < 0 ? ~s : 0;
+    int z = xo == 0 ? (bq_uniform(r) > qz) : 1;
+    if (!z) return 0;
+    int xp = bq_rnbinom_xz(r, mu, phi);
+    double lr = bq_neg_logit_sp(g0, g1, (double)xo) - bq_neg_logit_sp(g0, g1, (double)xp);
+    int xi = bq_log(bq_uniform(r)) <= lr ? xp : xo;
+    return ~xi;
+}
+
 /* per-histogram-bin gamma terms: sp(g0 + g1 x) */
 BQ_FN double bq_sp_lin(double g0, double g1, double x) { return bq_softplus(g0 + g1 * x); }
 

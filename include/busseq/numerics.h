@@ -17,6 +17,8 @@
  *   normal deviates      inverse CDF, Wichura 1988, Algorithm AS 241 (PPND16).
  *   gamma deviates       Marsaglia & Tsang 2000, with the U^(1/a) boost for a<1.
  *   Poisson deviates     sequential inversion (lambda < 10), Hoermann 1993 PTRS.
+ *   negative binomial    sequential inversion of the CDF (small means), else
+ *                        the gamma-Poisson mixture.
  *   log / exp            Cody-Waite argument reduction; polynomials fitted by
  *                        busseq_gpu_amd/csrc/gen_coeffs.py (Chebyshev, mpmath).
  *   lgamma               shift recurrence + Stirling series (Bernoulli numbers).
@@ -34,6 +36,16 @@
 #define BQ_FN static inline __host__ __device__ __attribute__((always_inline))
 #else
 #define BQ_FN static inline __attribute__((always_inline))
+#endif
+
+/* An int that holds the same value in every active lane of a wave (the step
+ * counter of a search loop that all lanes enter together).  Device code reads
+ * it from the first active lane so that table reads indexed by it become
+ * scalar loads; host code uses it as is.  The value is the same either way. */
+#ifdef __HIP_DEVICE_COMPILE__
+#define BQ_WAVE_UNIFORM(k) __builtin_amdgcn_readfirstlane(k)
+#else
+#define BQ_WAVE_UNIFORM(k) (k)
 #endif
 
 #define BQ_INF (__builtin_inf())
@@ -489,7 +501,7 @@ BQ_FN int bq_rpois(bq_rng* r, double lam)
         double s = p;
         double u = bq_uniform(r);
         int k = 0;
-        while (u > s && k < 200) { k += 1; p *= lam * bq_inv_tab[k]; s += p; }
+        while (u > s && k < 200) { k += 1; p *= lam * bq_inv_tab[BQ_WAVE_UNIFORM(k)]; s += p; }
         return k;
     }
     if (lam > 1.0e9) lam = 1.0e9;
@@ -520,6 +532,41 @@ BQ_FN int bq_rnbinom(bq_rng* r, double mu, double phi)
 {
     double lam = bq_rgamma(r, phi, mu / phi);
     return bq_rpois(r, lam);
+}
+
+/*
+ * Negative binomial NB(mean mu, size phi) by sequential inversion of its CDF:
+ *   p_0 = (phi / (phi + mu))^phi,  p_k = p_{k-1} q (phi + k - 1) / k,  q = mu / (phi + mu),
+ * returning the first k with u <= p_0 + ... + p_k.  One uniform per draw.
+ * bq_nb_inv_ok admits mu <= 30 with q <= 0.85; there P(X > 255) < 2e-13, far
+ * below 2^-33, the smallest distance of a 32-bit uniform from 1, and the
+ * search ends by k = 210 even at that extreme u (with a relative rounding
+ * error of the partial sums below 1e-11), so the k < 255 guard never binds.
+ */
+BQ_FN int bq_nb_inv_ok(double mu, double phi) { return mu <= 30.0 && 3.0 * mu <= 17.0 * phi; }
+
+BQ_FN int bq_rnbinom_inv(double u, double mu, double phi)
+{
+    double q = mu / (phi + mu);
+    double p = bq_exp(phi * bq_log1p(-q));
+    double s = p;
+    int k = 0;
+    while (u > s && k < 255) {
+        k += 1;
+        p *= q * ((phi + (double)(k - 1)) * bq_inv_tab[BQ_WAVE_UNIFORM(k)]);
+        s += p;
+    }
+    return k;
+}
+
+/* The proposal draw of the X/Z step: the first uniform of the stream drives
+ * the inversion where bq_nb_inv_ok holds; elsewhere it is skipped and the
+ * gamma-Poisson mixture runs on the following draws. */
+BQ_FN int bq_rnbinom_xz(bq_rng* r, double mu, double phi)
+{
+    double u = bq_uniform(r);
+    if (bq_nb_inv_ok(mu, phi)) return bq_rnbinom_inv(u, mu, phi);
+    return bq_rnbinom(r, mu, phi);
 }
 
 /* Beta(a, b) as a ratio of gamma deviates (cu:212-218) */

@@ -364,17 +364,6 @@ static void build_tiles(orc_t* o)
  *   log u <= log(1+e^{-g0-g1 x}) - log(1+e^{-g0-g1 x'})   (log form of cu:504);
  * if Z = 0 then X = 0.  Returns the new encoded state.
  */
-static int32_t xz_update(int32_t s, double mu, double phi, double g0, double g1, double qz, bq_rng* r)
-{
-    int xo = s < 0 ? ~s : 0;
-    int z = xo == 0 ? (bq_uniform(r) > qz) : 1;
-    if (!z) return 0;
-    int xp = bq_rnbinom(r, mu, phi);
-    double lr = bq_neg_logit_sp(g0, g1, (double)xo) - bq_neg_logit_sp(g0, g1, (double)xp);
-    int xi = bq_log(bq_uniform(r)) <= lr ? xp : xo;
-    return ~xi;
-}
-
 #define DEFINE_PASSES(T, SFX)                                                              \
 /* subtype draw: categorical full conditional of W_i (replaces MH cu:820-854) */           \
 static void pass_w_##SFX(orc_t* o, int t)                                                  \
@@ -457,7 +446,7 @@ static void pass_xz_alpha_##SFX(orc_t* o, int t)                                
                 int xi;                                                                    \
                 if (s <= 0) {                                                              \
                     bq_rng r; bq_rng_init(&r, o->seed, o->chain, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ); \
-                    int32_t sn = xz_update(s, (double)mu, o->phi[bg], g0, g1, qz[b], &r);  \
+                    int32_t sn = bq_xz_update(s, (double)mu, o->phi[bg], g0, g1, qz[b], &r);  \
                     o->S[e] = sn;                                                          \
                     xi = sn < 0 ? ~sn : 0;                                                 \
                     if (sn < 0) { sm[0 * B + b] += 1; sm[1 * B + b] += (uint64_t)xi; }    \
@@ -1001,7 +990,7 @@ void orc_philox(const uint32_t* c, const uint32_t* k, uint32_t* out)
 }
 
 /* kind: 0 uniform, 1 normal, 2 gamma(p1, scale p2), 3 poisson(p1), 4 nbinom(mu p1, phi p2),
- *       5 beta(p1, p2), 6 raw u32 words */
+ *       5 beta(p1, p2), 6 raw u32 words, 7 the X/Z proposal draw nbinom_xz(mu p1, phi p2) */
 void orc_stream(uint32_t seed, uint32_t chain, uint32_t a, uint32_t b, uint32_t sweep,
                 uint32_t phase, int kind, double p1, double p2, int n, double* out)
 {
@@ -1014,9 +1003,16 @@ void orc_stream(uint32_t seed, uint32_t chain, uint32_t a, uint32_t b, uint32_t 
         case 3: out[j] = (double)bq_rpois(&r, p1); break;
         case 4: out[j] = (double)bq_rnbinom(&r, p1, p2); break;
         case 5: out[j] = bq_rbeta(&r, p1, p2); break;
+        case 7: out[j] = (double)bq_rnbinom_xz(&r, p1, p2); break;
         default: out[j] = (double)bq_next32(&r); break;
         }
     }
+}
+
+/* negative-binomial inversion at given uniforms (admissible (mu, phi) only) */
+void orc_nb_inv(const double* u, const double* mu, const double* phi, double* out, int n)
+{
+    for (int j = 0; j < n; ++j) out[j] = (double)bq_rnbinom_inv(u[j], mu[j], phi[j]);
 }
 
 /* fn: 0 log, 1 log1p, 2 exp, 3 expm1, 4 lgamma, 5 lgamma_diff64(x,y), 6 logf, 7 log1pf,
@@ -1050,7 +1046,7 @@ void orc_xz_chain(double mu, double phi, double g0, double g1, uint32_t seed, in
     double qz = 1.0 / (1.0 + bq_exp(g0));
     for (int t = 0; t < n; ++t) {
         bq_rng r; bq_rng_init(&r, seed, 7u, 0u, 0u, (uint32_t)t, BQ_PH_XZ);
-        s = xz_update(s, mu, phi, g0, g1, qz, &r);
+        s = bq_xz_update(s, mu, phi, g0, g1, qz, &r);
         int x = s < 0 ? ~s : 0, z = s < 0;
         if (x < nx) counts[x * 2 + z] += 1;
     }

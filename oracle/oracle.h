@@ -77,6 +77,7 @@ void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
 void orc_stream(uint32_t seed, uint32_t chain, uint32_t a, uint32_t b, uint32_t sweep,
                 uint32_t phase, int kind, double p1, double p2, int n, double* out);
 void orc_math(int fn, const double* x, const double* y, double* out, int n);
+void orc_nb_inv(const double* u, const double* mu, const double* phi, double* out, int n);
 void orc_terms(int fn, int prec, const double* args, int nargs, double* out, int n);
 void orc_xz_chain(double mu, double phi, double g0, double g1, uint32_t seed, int n, int nx, int64_t* counts);
 

@@ -47,6 +47,7 @@ def load():
     lib.orc_philox.argtypes = [_P(np.uint32), _P(np.uint32), _P(np.uint32)]
     lib.orc_stream.argtypes = [C.c_uint32] * 6 + [C.c_int, C.c_double, C.c_double, C.c_int, _P(np.float64)]
     lib.orc_math.argtypes = [C.c_int, _P(np.float64), _P(np.float64), _P(np.float64), C.c_int]
+    lib.orc_nb_inv.argtypes = [_P(np.float64)] * 4 + [C.c_int]
     lib.orc_terms.argtypes = [C.c_int, C.c_int, _P(np.float64), C.c_int, _P(np.float64), C.c_int]
     lib.orc_xz_chain.argtypes = [C.c_double] * 4 + [C.c_uint32, C.c_int, C.c_int, _P(np.int64)]
     _lib = lib
@@ -65,7 +66,8 @@ POST_KEYS = ("gamma", "alpha", "L", "beta", "ptau", "nu", "delta", "phi", "pi", 
 # names of orc_math functions
 MATH = {"log": 0, "log1p": 1, "exp": 2, "expm1": 3, "lgamma": 4, "lgamma_diff64": 5, "logf": 6,
         "log1pf": 7, "lgamma_diff32": 8, "softplus": 9, "ndtri": 10, "fix": 11}
-STREAM = {"uniform": 0, "normal": 1, "gamma": 2, "poisson": 3, "nbinom": 4, "beta": 5, "u32": 6}
+STREAM = {"uniform": 0, "normal": 1, "gamma": 2, "poisson": 3, "nbinom": 4, "beta": 5, "u32": 6,
+          "nbinom_xz": 7}
 
 
 def math(fn, x, y=None):
@@ -81,6 +83,15 @@ def stream(kind, n, seed=1, chain=0, a=0, b=0, sweep=0, phase=10, p1=0.0, p2=0.0
     lib = load()
     out = np.zeros(n)
     lib.orc_stream(seed, chain, a, b, sweep, phase, STREAM[kind], p1, p2, n, out)
+    return out
+
+
+def nb_inv(u, mu, phi):
+    """bq_rnbinom_inv at the given uniforms"""
+    lib = load()
+    u, mu, phi = (np.ascontiguousarray(a, np.float64) for a in np.broadcast_arrays(u, mu, phi))
+    out = np.zeros_like(u)
+    lib.orc_nb_inv(u, mu, phi, out, len(u))
     return out
 
 

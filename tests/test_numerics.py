@@ -170,9 +170,13 @@ def test_poisson_distribution(lam):
     assert abs(x.mean() - lam) < 5 * np.sqrt(lam / len(x))
 
 
-@pytest.mark.parametrize("mu,phi", [(0.5, 5.0), (20.0, 5.0), (300.0, 0.7), (5.0, 0.05)])
-def test_negative_binomial_distribution(mu, phi):
-    x = draws("nbinom", 40000, mu, phi).astype(int)
+NB_CASES = [(0.5, 5.0), (20.0, 5.0), (29.9, 5.3), (300.0, 0.7), (5.0, 0.05), (31.0, 50.0)]
+
+
+@pytest.mark.parametrize("kind", ["nbinom", "nbinom_xz"])
+@pytest.mark.parametrize("mu,phi", NB_CASES)
+def test_negative_binomial_distribution(kind, mu, phi):
+    x = draws(kind, 40000, mu, phi).astype(int)
     dist = st.nbinom(phi, phi / (phi + mu))
     assert abs(x.mean() - mu) < 6 * np.sqrt(dist.var() / len(x))
     hi = int(dist.ppf(0.999))
@@ -183,6 +187,27 @@ def test_negative_binomial_distribution(mu, phi):
     keep = exp > 5
     chi = ((obs[keep] - exp[keep]) ** 2 / exp[keep]).sum()
     assert st.chi2.sf(chi, keep.sum() - 1) > 1e-3
+
+
+def test_negative_binomial_inversion_is_the_exact_quantile():
+    # bq_rnbinom_inv(u) must be the smallest k with CDF(k) >= u (scipy's ppf) over the
+    # admissible region mu <= 30, mu/(phi+mu) <= 0.85, including u next to 1
+    rng = np.random.default_rng(5)
+    n = 3000
+    mu = np.exp(rng.uniform(np.log(1e-3), np.log(30.0), n))
+    phi = mu * (0.15 / 0.85) * (1 + 1e-9) * np.exp(rng.uniform(0, np.log(1e4), n))
+    u = (rng.integers(0, 2 ** 32, n) + 0.5) * 2.0 ** -32
+    u[:20] = 1 - 2.0 ** -33
+    assert ((mu <= 30) & (3 * mu <= 17 * phi)).all()
+    got = po.nb_inv(u, mu, phi)
+    ref = st.nbinom.ppf(u, phi, phi / (phi + mu))
+    assert got.max() <= 255
+    # a draw may differ only where u lies within rounding of a CDF step
+    bad = got != ref
+    step = np.abs(st.nbinom.cdf(np.minimum(got, ref), phi, phi / (phi + mu)) - u)
+    assert bad.sum() <= 2 and (step[bad] < 1e-12).all(), (got[bad], ref[bad])
+    # planted mistake: a mean off by 5% is caught
+    assert (po.nb_inv(u, mu * 1.05, phi) != ref).sum() > 50
 
 
 @pytest.mark.parametrize("a,b", [(2.0, 5.0), (1.0, 3.0), (40.0, 3000.0)])

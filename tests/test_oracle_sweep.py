@@ -237,13 +237,22 @@ def test_tile_rows_change_only_rounding():
 
 @pytest.mark.slow
 def test_recovers_synthetic_truth():
+    # A single short chain can settle in a mode with two subtypes merged or one
+    # emptied (the posterior is multimodal); the chain of lowest BIC among four
+    # recovers the truth, and every chain that missed it has a higher BIC.
     from sklearn.metrics import adjusted_rand_score
     n_b = [60, 50, 40]
     Y, W, X = bq.simulate(n_b, 300, 3, 2024, want_truth=True)
-    o = po.Oracle(Y, n_b, 3, seed=13579, precision=32, n_iter=300, n_burnin=150, n_unchanged=50)
-    o.sweeps(300)
-    post = o.posterior()
-    assert adjusted_rand_score(W, post["W"]) > 0.95
+    runs = []
+    for chain in range(4):
+        o = po.Oracle(Y, n_b, 3, seed=13579, chain=chain, precision=32, n_iter=300, n_burnin=150,
+                      n_unchanged=50)
+        o.sweeps(300)
+        post = o.posterior()
+        runs.append((post["bic"], adjusted_rand_score(W, post["W"]), o))
+    bic, ari, o = min(runs, key=lambda r: r[0])
+    assert ari > 0.95
+    assert all(r[0] > bic for r in runs if r[1] < 0.9)
     s = o.state()
     # imputed true counts at the dropout positions track the hidden truth
     zeros = Y == 0
