@@ -43,7 +43,7 @@
     Q double *alpha, *beta, *nu, *delta, *phi, *gamma, *pi, *ptau; \
     Q double *mu_alpha, *mu_nu, *mu_delta; \
     /* precision-T copies of the parameters */ \
-    Q void *ea, *eb, *em, *bt, *en, *ed, *ph; \
+    Q void *ea, *eb, *bt, *en, *ed, *ph;  /* bt: [K*G][2] (beta, expm1 beta) */ \
     /* proposals and their precision-T constants */ \
     Q double *alpha_p, *beta_p, *nu_p, *phi_p, *gamma_p; \
     Q void *da, *ca, *db, *cb, *dn, *cn; \

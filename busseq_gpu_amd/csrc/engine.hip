@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/busseq.h"
@@ -148,6 +149,7 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
 template <typename T, int KT>
 __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
 {
+    typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -157,29 +159,30 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
     const T* ea = (const T*)c.ea;
     const T* en = (const T*)c.en + (size_t)b * G;
     const T* ph = (const T*)c.ph + (size_t)b * G;
-    const T* bt = (const T*)c.bt;
-    const T* em = (const T*)c.em;
+    const T2* be = (const T2*)c.bt;          // (beta_kg, expm1(beta_kg)) pairs
     const T edi = ((const T*)c.ed)[i];
-    auto row = c.S + (size_t)i * c.Gp;
+    const int32_t* row = (const int32_t*)(c.S + (size_t)i * c.Gp);
 
     uint64_t acc[KT];
 #pragma unroll
     for (int k = 0; k < KT; ++k) acc[k] = 0;
     for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
-        const int c1 = c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G;
+        const unsigned c1 = (unsigned)(c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G);
         double part[KT];
 #pragma unroll
         for (int k = 0; k < KT; ++k) part[k] = 0.0;
-        for (int g = c0 + lane; g < c1; g += 64) {
+        for (unsigned g = (unsigned)(c0 + lane); g < c1; g += 64) {
             int32_t s = row[g];
             int xi = s < 0 ? ~s : s;
             T x = (T)xi;
             T phi = ph[g];
+            T2 v[KT];
+#pragma unroll
+            for (int k = 1; k < KT; ++k) v[k] = be[(unsigned)k * (unsigned)G + g];
             T mu0 = tmu(ea[g], (T)1, en[g], edi);
             T r0 = tratio(mu0, phi);
 #pragma unroll
-            for (int k = 1; k < KT; ++k)
-                part[k] += (double)ttw(x, phi, r0, bt[(size_t)k * G + g], em[(size_t)k * G + g]);
+            for (int k = 1; k < KT; ++k) part[k] += (double)ttw(x, phi, r0, v[k].x, v[k].y);
         }
 #pragma unroll
         for (int k = 1; k < KT; ++k) acc[k] += bq_fix(part[k]);
@@ -834,8 +837,8 @@ __global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
     if (acc) {
         c.beta[q] = bp;
         ((T*)c.eb)[q] = (T)bq_exp(bp);
-        ((T*)c.em)[q] = (T)bq_expm1(bp);
-        ((T*)c.bt)[q] = (T)bp;
+        ((T*)c.bt)[2 * q] = (T)bp;
+        ((T*)c.bt)[2 * q + 1] = (T)bq_expm1(bp);
     }
 }
 
@@ -1197,7 +1200,15 @@ void to_t(std::vector<char>& out, const std::vector<double>& v, double (*f)(doub
     for (size_t j = 0; j < v.size(); ++j) p[j] = (T)(f ? f(v[j]) : v[j]);
 }
 double f_exp(double x) { return bq_exp(x); }
-double f_expm1(double x) { return bq_expm1(x); }
+
+// (beta, expm1(beta)) pairs in precision T, the layout k_w reads
+template <typename T>
+void to_pairs(std::vector<char>& out, const std::vector<double>& v)
+{
+    out.resize(2 * v.size() * sizeof(T));
+    T* p = (T*)out.data();
+    for (size_t j = 0; j < v.size(); ++j) { p[2 * j] = (T)v[j]; p[2 * j + 1] = (T)bq_expm1(v[j]); }
+}
 
 // precision-T copies of a chain's parameters, computed on the host with the
 // same functions the kernels use
@@ -1211,16 +1222,14 @@ busseq_status upload_tcopies(busseq_sampler* s, int ch, const std::vector<double
     if (s->prec == 32) {
         to_t<float>(buf, alpha, f_exp); HIPCHK(up(c.ea));
         to_t<float>(buf, beta, f_exp); HIPCHK(up(c.eb));
-        to_t<float>(buf, beta, f_expm1); HIPCHK(up(c.em));
-        to_t<float>(buf, beta, nullptr); HIPCHK(up(c.bt));
+        to_pairs<float>(buf, beta); HIPCHK(up(c.bt));
         to_t<float>(buf, nu, f_exp); HIPCHK(up(c.en));
         to_t<float>(buf, delta, f_exp); HIPCHK(up(c.ed));
         to_t<float>(buf, phi, nullptr); HIPCHK(up(c.ph));
     } else {
         to_t<double>(buf, alpha, f_exp); HIPCHK(up(c.ea));
         to_t<double>(buf, beta, f_exp); HIPCHK(up(c.eb));
-        to_t<double>(buf, beta, f_expm1); HIPCHK(up(c.em));
-        to_t<double>(buf, beta, nullptr); HIPCHK(up(c.bt));
+        to_pairs<double>(buf, beta); HIPCHK(up(c.bt));
         to_t<double>(buf, nu, f_exp); HIPCHK(up(c.en));
         to_t<double>(buf, delta, f_exp); HIPCHK(up(c.ed));
         to_t<double>(buf, phi, nullptr); HIPCHK(up(c.ph));
@@ -1355,8 +1364,8 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.phi = s->dalloc<double>((size_t)B * G); c.gamma = s->dalloc<double>(2 * (size_t)B);
             c.pi = s->dalloc<double>((size_t)B * K); c.ptau = s->dalloc<double>(2);
             c.mu_alpha = s->dalloc<double>(G); c.mu_nu = s->dalloc<double>(B); c.mu_delta = s->dalloc<double>(N);
-            c.ea = s->dalloc<char>(ts * G); c.eb = s->dalloc<char>(ts * K * G); c.em = s->dalloc<char>(ts * K * G);
-            c.bt = s->dalloc<char>(ts * K * G); c.en = s->dalloc<char>(ts * B * G); c.ed = s->dalloc<char>(ts * N);
+            c.ea = s->dalloc<char>(ts * G); c.eb = s->dalloc<char>(ts * K * G);
+            c.bt = s->dalloc<char>(2 * ts * K * G); c.en = s->dalloc<char>(ts * B * G); c.ed = s->dalloc<char>(ts * N);
             c.ph = s->dalloc<char>(ts * B * G);
             c.alpha_p = s->dalloc<double>(G); c.beta_p = s->dalloc<double>((size_t)K * G);
             c.nu_p = s->dalloc<double>((size_t)B * G); c.phi_p = s->dalloc<double>((size_t)B * G);

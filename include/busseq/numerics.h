@@ -339,17 +339,33 @@ BQ_FN float bq_logf(float x)
     return res;
 }
 
+/* log1p for the arguments the fast path does not take: y <= -1, NaN, +inf and
+ * 1 + y >= sqrt(2) 2^125 (k >= 126), where 1 + y equals y up to a relative 2^-125 */
+BQ_FN float bq_log1pf_rare(float y)
+{
+    if (!(y > -1.0f)) return y == -1.0f ? -BQ_INFF : BQ_NANF;
+    return bq_logf(1.0f + y);
+}
+
+/*
+ * log1p(y) in single precision.  1 + y = 2^k m with m in [sqrt(1/2), sqrt(2)),
+ * k taken from the rounded sum t = 1 + y; the reduced argument
+ *   f = (1 + y) 2^-k - 1 = fma(y, 2^-k, 2^-k - 1)
+ * is formed from the unrounded y with a single rounding (2^-k - 1 is exact
+ * for the k that occur here or carries an error far below f's own), so no
+ * correction term for the rounding of 1 + y is needed, and for |y| small
+ * (k = 0) f = y exactly.
+ */
 BQ_FN float bq_log1pf(float y)
 {
-    int inr = y >= BQ_LOG1P_LO_F && y <= BQ_LOG1P_HI_F;
-    int e;
-    float m = bq_split32(1.0f + y, &e);
-    float f = inr ? y : m - 1.0f;
-    float ef = inr ? 0.0f : (float)e;
+    float t = 1.0f + y;
+    uint32_t kb = (bq_fbits(t) - 0x3F3504F3u) & 0xFF800000u;   /* k << 23 */
+    float s = bq_bitsf(0x3F800000u - kb);                        /* 2^-k */
+    float f = __builtin_fmaf(y, s, s - 1.0f);
+    float ef = (float)((int32_t)kb >> 23);
     float r = f + (f * f) * bq_log1p_q32(f);
     float res = ef * BQ_LN2_HI_F + (ef * BQ_LN2_LO_F + r);
-    if (!(y > -1.0f)) res = y == -1.0f ? -BQ_INFF : BQ_NANF;
-    if (y == BQ_INFF) res = BQ_INFF;
+    if (!(y > -1.0f && t < 0x1.6a09e6p+125f)) res = bq_log1pf_rare(y);   /* k <= 125 */
     return res;
 }
 

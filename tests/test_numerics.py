@@ -104,6 +104,22 @@ def test_single_precision_logs_vs_mpmath(gold):
     assert ulps(po.math("log1pf", y.astype(np.float64)), ref, np.float32).max() <= 3.0
 
 
+def test_single_precision_log1p_over_its_reduction_ranges():
+    # every exponent k of 1 + y from -24 to 127, |y| tiny (f = y exactly), and both
+    # sides of the switch to the rare path at 1 + y = sqrt(2) 2^125
+    import mpmath as mp
+    rng = np.random.default_rng(9)
+    y = np.concatenate([
+        -1 + np.exp2(-rng.uniform(0, 24, 400)),
+        np.exp2(rng.uniform(-40, 127.9, 800)),
+        -np.exp2(rng.uniform(-40, -0.1, 300)),
+        np.float32(2 ** 125.5) * (1 + rng.uniform(-1e-6, 1e-6, 50)),
+    ]).astype(np.float32)
+    y = y[np.isfinite(y) & (y > -1)]
+    ref = np.array([float(mp.log1p(mp.mpf(float(v)))) for v in y])
+    assert ulps(po.math("log1pf", y.astype(np.float64)), ref, np.float32).max() <= 2.0
+
+
 def test_ndtri_vs_mpmath(gold):
     y = gold["ndtri_y"]
     err = np.abs(po.math("ndtri", gold["ndtri_x"]) - y) / np.maximum(np.abs(y), 1.0)
