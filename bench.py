@@ -54,15 +54,16 @@ def kernel_bytes(name, N, G, n_zero):
     return None
 
 
-def pmc_traffic(kernel, cfg_name):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+def pmc_profile(kernel, cfg_name, key):
+    """A per-launch figure of the committed rocprofv3 PMC summary (HBM bytes per
+    launch from FETCH_SIZE/WRITE_SIZE, or the VALU share of peak), if present."""
     path = os.path.join(ROOT, "profiles", f"pmc_traffic_{cfg_name}.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get("kernels", {}).get(kernel, {}).get("bytes_per_launch")
+        return d.get("kernels", {}).get(kernel, {}).get(key)
     except (OSError, ValueError):
         return None
 
@@ -164,10 +165,12 @@ def main():
     roof = None
     if dom_bytes is not None:
         ach = dom_bytes / (kt[dom] * 1e-3) / 1e9
-        traffic = pmc_traffic(dom, args.config)
+        traffic = pmc_profile(dom, args.config, "bytes_per_launch")
+        valu = pmc_profile(dom, args.config, "valu_share")
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
-                "traffic": traffic, "algorithmic_bytes": dom_bytes, "avg_ms": round(kt[dom], 5)}
+                "traffic": traffic, "algorithmic_bytes": dom_bytes, "avg_ms": round(kt[dom], 5),
+                "valu_share_profiled": round(valu, 3) if valu is not None else None}
     sweep_bytes = 16.0 * N * G
     s.close()
 

@@ -13,6 +13,7 @@
 #define BQ_BMAX 128    // largest B (the reference's __constant__ limit, cu:9-10)
 #define BQ_WAVE 64
 #define BQ_CHUNK 1024 // genes per row-sum chunk (part of the reduction contract)
+#define BQ_TR_MAX 256  // largest tile (cells per tile-pass workgroup)
 #define BQ_HLO 64      // histogram bins kept in striped copies (the frequent small values)
 #define BQ_STRIPES 32 // copies of each per-sweep counter, one per group of workgroups, so that
                       // the atomics of thousands of cells do not queue on one cache line

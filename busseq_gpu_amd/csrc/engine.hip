@@ -1270,6 +1270,7 @@ busseq_status validate(const busseq_config* cfg)
     if (cfg->n_iter < 0 || cfg->n_burnin < 0) { bq_set_error("negative iteration count"); return BUSSEQ_ERR_ARG; }
     if (cfg->precision != 32 && cfg->precision != 64) { bq_set_error("precision must be 32 or 64"); return BUSSEQ_ERR_ARG; }
     if (cfg->chains < 1) { bq_set_error("chains must be >= 1"); return BUSSEQ_ERR_ARG; }
+    if (cfg->tile_rows < 0 || cfg->tile_rows > BQ_TR_MAX) { bq_set_error("tile_rows must be between 1 and 256 (0: default)"); return BUSSEQ_ERR_ARG; }
     long long N = 0;
     for (int b = 0; b < cfg->B; ++b) N += cfg->n_b[b];
     if (N * (long long)((cfg->G + 63) & ~63) > 2147483647LL) { bq_set_error("N x G must stay below 2^31"); return BUSSEQ_ERR_ARG; }

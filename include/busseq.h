@@ -49,7 +49,7 @@ typedef struct busseq_config {
     int precision;        /* 32: single-precision per-element terms (default); 64: double */
     int chains;           /* independent chains on the same data in one launch (default 1) */
     int chain0;           /* Philox key word of the first chain (default 0) */
-    int tile_rows;        /* cells per reduction tile (default 64) */
+    int tile_rows;        /* cells per reduction tile, 1..256 (0 or default: 64) */
     int device;           /* HIP device ordinal (default 0) */
 } busseq_config;
 
