@@ -68,7 +68,7 @@
     Q int* n_rec; \
     Q double *s_gamma, *s_alpha, *s_beta, *s_nu, *s_delta, *s_phi, *s_pi, *s_ptau; \
     Q int32_t *c_L, *c_W; \
-    /* traces [param * n_pres + r] */ \
+    /* traces, iteration-major: [r * n_params + param] */ \
     Q double *tr_gamma, *tr_alpha, *tr_beta, *tr_ptau, *tr_nu, *tr_delta, *tr_phi, *tr_pi; \
     Q int32_t *tr_L, *tr_W; \
     /* end */

@@ -575,6 +575,20 @@ __global__ __launch_bounds__(256) void k_alpha(const ChainDev* __restrict__ cds)
     c.part[((size_t)tile * 3) * c.Gp + g] = bq_fix(acc);
 }
 
+// exact sum of the fixed-point tile partials p[tt * stride], tt in [t0, t1),
+// with eight loads in flight (wrapping integer adds: the order does not matter)
+__device__ __forceinline__ uint64_t sum_tiles(const uint64_t* p, int t0, int t1, size_t stride)
+{
+    uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int tt = t0;
+    for (; tt + 8 <= t1; tt += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += p[(size_t)(tt + u) * stride];
+    }
+    for (; tt < t1; ++tt) a[0] += p[(size_t)tt * stride];
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 // ---------------------------------------------------------------------------
 // k_alpha_dec: flat work items
 //   [0, G)                 alpha accept (cu:625-632)
@@ -591,18 +605,7 @@ __global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ 
     const int t = *c.iter;
     if (j < G) {
         const int g = j, nt = *c.ntiles;
-        const size_t stride = (size_t)3 * c.Gp;
-        auto pp = c.part + g;
-        uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        int tt = 0;
-        for (; tt + 4 <= nt; tt += 4) {
-            a0 += pp[(size_t)tt * stride];
-            a1 += pp[(size_t)(tt + 1) * stride];
-            a2 += pp[(size_t)(tt + 2) * stride];
-            a3 += pp[(size_t)(tt + 3) * stride];
-        }
-        for (; tt < nt; ++tt) a0 += pp[(size_t)tt * stride];
-        double sum = bq_unfix((a0 + a1) + (a2 + a3));
+        double sum = bq_unfix(sum_tiles((const uint64_t*)(c.part + g), 0, nt, (size_t)3 * c.Gp));
         double al = c.alpha[g], ap = c.alpha_p[g];
         double lr = sum + bq_lr_normal(al, ap, c.mu_alpha[g], BQ_SIGMA2_ALPHA);
         if (bq_log(ph_uniform(c, (uint32_t)g, 0u, t, BQ_PH_ALPHA_A)) <= lr) {
@@ -824,10 +827,8 @@ __global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
     const size_t Gp = c.Gp;
     auto toff = c.tile_off;
     uint64_t a = 0;
-    for (int b = 0; b < B; ++b) {
-        int e = toff[b * K + k + 1];
-        for (int tt = toff[b * K + k]; tt < e; ++tt) a += c.part[((size_t)tt * 3) * Gp + g];
-    }
+    for (int b = 0; b < B; ++b)
+        a += sum_tiles((const uint64_t*)(c.part + g), toff[b * K + k], toff[b * K + k + 1], 3 * Gp);
     const size_t q = (size_t)k * G + g;
     double sum = bq_unfix(a);
     double be = c.beta[q], bp = c.beta_p[q];
@@ -859,8 +860,7 @@ __global__ __launch_bounds__(256) void k_n_dec(const ChainDev* __restrict__ cds)
     uint64_t a = 0;
     for (int k = 0; k < K; ++k) {
         const int slot = (k >= 1 && c.beta_acc[(size_t)k * G + g]) ? 2 : 1;
-        const int e = toff[b * K + k + 1];
-        for (int tt = toff[b * K + k]; tt < e; ++tt) a += c.part[((size_t)tt * 3 + slot) * Gp + g];
+        a += sum_tiles((const uint64_t*)(c.part + slot * Gp + g), toff[b * K + k], toff[b * K + k + 1], 3 * Gp);
     }
     const size_t q = (size_t)b * G + g;
     double sum = bq_unfix(a);
@@ -969,9 +969,7 @@ __global__ __launch_bounds__(256) void k_phi_dec(const ChainDev* __restrict__ cd
     if (j >= G * B) return;
     const int b = j / G, g = j % G, t = *c.iter;
     const size_t Gp = c.Gp;
-    uint64_t a = 0;
-    const int e = c.tile_off[(b + 1) * K];
-    for (int tt = c.tile_off[b * K]; tt < e; ++tt) a += c.part[((size_t)tt * 3) * Gp + g];
+    const uint64_t a = sum_tiles((const uint64_t*)(c.part + g), c.tile_off[b * K], c.tile_off[(b + 1) * K], 3 * Gp);
     const size_t q = (size_t)b * G + g;
     double sum = bq_unfix(a);
     double f = c.phi[q], fp = c.phi_p[q];
@@ -999,43 +997,43 @@ __global__ __launch_bounds__(256) void k_record(const ChainDev* __restrict__ cds
     if (j == 0) *c.n_rec += 1;
     if (j < 2 * B) {
         c.s_gamma[j] += c.gamma[j];
-        if (tr) c.tr_gamma[(size_t)j * P + r] = c.gamma[j];
+        if (tr) c.tr_gamma[(size_t)r * (2 * B) + j] = c.gamma[j];
     }
     if (j < 2) {
         c.s_ptau[j] += c.ptau[j];
-        if (tr) c.tr_ptau[(size_t)j * P + r] = c.ptau[j];
+        if (tr) c.tr_ptau[(size_t)r * 2 + j] = c.ptau[j];
     }
     if (j < G) {
         c.s_alpha[j] += c.alpha[j];
-        if (tr) c.tr_alpha[(size_t)j * P + r] = c.alpha[j];
+        if (tr) c.tr_alpha[(size_t)r * G + j] = c.alpha[j];
     }
     if (j < K * G) {
         c.s_beta[j] += c.beta[j];
         c.c_L[j] += c.L[j];
         if (tr && j >= G) {
-            c.tr_beta[(size_t)(j - G) * P + r] = c.beta[j];
-            c.tr_L[(size_t)(j - G) * P + r] = c.L[j];
+            c.tr_beta[(size_t)r * ((K - 1) * G) + (j - G)] = c.beta[j];
+            c.tr_L[(size_t)r * ((K - 1) * G) + (j - G)] = c.L[j];
         }
     }
     if (j < B * G) {
         c.s_nu[j] += c.nu[j];
         c.s_phi[j] += c.phi[j];
         if (tr) {
-            if (j >= G) c.tr_nu[(size_t)(j - G) * P + r] = c.nu[j];
-            c.tr_phi[(size_t)j * P + r] = c.phi[j];
+            if (j >= G) c.tr_nu[(size_t)r * ((B - 1) * G) + (j - G)] = c.nu[j];
+            c.tr_phi[(size_t)r * (B * G) + j] = c.phi[j];
         }
     }
     if (j < N) {
         c.s_delta[j] += c.delta[j];
         c.c_W[(size_t)j * K + c.W[j]] += 1;
         if (tr) {
-            c.tr_delta[(size_t)j * P + r] = c.delta[j];
-            c.tr_W[(size_t)j * P + r] = c.W[j];
+            c.tr_delta[(size_t)r * N + j] = c.delta[j];
+            c.tr_W[(size_t)r * N + j] = c.W[j];
         }
     }
     if (j < B * K) {
         c.s_pi[j] += c.pi[j];
-        if (tr) c.tr_pi[(size_t)j * P + r] = c.pi[j];
+        if (tr) c.tr_pi[(size_t)r * (B * K) + j] = c.pi[j];
     }
 }
 
@@ -1720,14 +1718,24 @@ extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result*
         r->bic = -2.0 * tot + bq_log((double)G * (double)N) * npar;
     }
     if (c.traces && r->n_pres > 0) {
+        // device traces are iteration-major ([iter][param], one coalesced row per
+        // recorded sweep); the result holds them parameter-major ([param][iter])
         const size_t P = c.n_pres;
         auto cpd = [&](double*& dst, const double* src, size_t k) -> hipError_t {
             dst = md(k * P);
-            return hipMemcpy(dst, src, k * P * 8, hipMemcpyDeviceToHost);
+            std::vector<double> tmp(k * P);
+            hipError_t e = hipMemcpy(tmp.data(), src, k * P * 8, hipMemcpyDeviceToHost);
+            for (size_t r = 0; r < P; ++r)
+                for (size_t j = 0; j < k; ++j) dst[j * P + r] = tmp[r * k + j];
+            return e;
         };
         auto cpi = [&](int32_t*& dst, const int32_t* src, size_t k) -> hipError_t {
             dst = mi(k * P);
-            return hipMemcpy(dst, src, k * P * 4, hipMemcpyDeviceToHost);
+            std::vector<int32_t> tmp(k * P);
+            hipError_t e = hipMemcpy(tmp.data(), src, k * P * 4, hipMemcpyDeviceToHost);
+            for (size_t r = 0; r < P; ++r)
+                for (size_t j = 0; j < k; ++j) dst[j * P + r] = tmp[r * k + j];
+            return e;
         };
         HIPCHK(cpd(r->tr_gamma, c.tr_gamma, 2 * (size_t)B));
         HIPCHK(cpd(r->tr_alpha, c.tr_alpha, G));

@@ -1,0 +1,84 @@
+"""Posterior-level checks on the GPU.
+
+Traces: every recorded iteration (the -p output of BUSseq_gpu.cu:892-936,
+1846-2018) averages to the posterior mean the sampler accumulates and ends in
+the chain's final state.
+
+Mixed precision against FP64 (the agreement the mixed-precision form must
+keep): on the c1 shape (B=4, N=1000, G=3000, K=5) four FP32 chains run in one
+batched sampler; the chain of lowest BIC and an FP64 chain with the same
+Philox key recover the true subtypes (ARI >= 0.99, also against each other),
+and their posterior means agree within 3 Monte-Carlo standard errors for all
+but a few percent of the parameters (standard errors from the effective
+sample size, Geyer's initial positive sequence).
+"""
+import numpy as np
+import pytest
+
+import busseq_gpu_amd as bq
+
+pytestmark = pytest.mark.gpu
+
+
+def test_traces_match_posterior_and_state():
+    n_b, G, K = [30, 25, 20], 200, 3
+    Y = bq.simulate(n_b, G, K, 77)
+    with bq.Sampler(Y, n_b, K, seed=5, n_iter=40, n_burnin=15, n_unchanged=5, keep_traces=True) as s:
+        s.sweeps(40)
+        st = s.state()
+        post = s.finish()
+    tr = post["traces"]
+    assert post["n_pres"] == 25 and tr["alpha"].shape == (G, 25)
+    for key in ("alpha", "delta", "pi", "gamma"):
+        np.testing.assert_allclose(tr[key].mean(1), np.ravel(post[key]), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(tr["beta"].mean(1), post["beta"][1:].ravel(), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(tr["nu"].mean(1), post["nu"][1:].ravel(), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(tr["phi"].mean(1), post["phi"].ravel(), rtol=1e-12, atol=1e-12)
+    assert np.array_equal(tr["alpha"][:, -1], st["alpha"])
+    assert np.array_equal(tr["delta"][:, -1], st["delta"])
+    assert np.array_equal(tr["W"][:, -1], st["W"])
+    assert np.array_equal(tr["beta"][:, -1], st["beta"][1:].ravel())
+    assert np.array_equal(tr["L"][:, -1], st["L"][1:].ravel())
+
+
+def mc_se(tr):
+    """Monte-Carlo standard error of each row's mean, tau from Geyer's initial positive sequence"""
+    x = tr - tr.mean(1, keepdims=True)
+    n = x.shape[1]
+    f = np.fft.rfft(x, 2 * n, axis=1)
+    acov = np.fft.irfft(f * np.conj(f), axis=1)[:, :n] / n
+    var = acov[:, 0]
+    rho = acov / np.maximum(var[:, None], 1e-300)
+    m = (n - 1) // 2
+    pairs = rho[:, 0:2 * m:2] + rho[:, 1:2 * m:2]
+    keep = np.cumprod(pairs > 0, axis=1)
+    tau = np.maximum(-1.0 + 2.0 * (pairs * keep).sum(1), 1.0)
+    return np.sqrt(var * tau / n)
+
+
+def test_mixed_precision_posterior_agrees_with_fp64():
+    from sklearn.metrics import adjusted_rand_score as ari
+    n_b, G, K = [300, 300, 200, 200], 3000, 5
+    Y, W_true, _ = bq.simulate(n_b, G, K, 1001, want_truth=True)
+    run = dict(seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500, keep_traces=True)
+    with bq.Sampler(Y, n_b, K, chains=4, precision=32, **run) as s:
+        s.sweeps(4000)
+        posts = [s.finish(c) for c in range(4)]
+    best = int(np.argmin([p["bic"] for p in posts]))
+    p32 = posts[best]
+    with bq.Sampler(Y, n_b, K, chain0=best, precision=64, **run) as s:
+        s.sweeps(4000)
+        p64 = s.finish()
+    print(f"\nchain {best}: ARI(truth) fp32 {ari(W_true, p32['W']):.4f} fp64 {ari(W_true, p64['W']):.4f} "
+          f"fp32-fp64 {ari(p32['W'], p64['W']):.4f}; BIC {p32['bic']:.1f} / {p64['bic']:.1f}")
+    assert ari(W_true, p32["W"]) >= 0.99 and ari(W_true, p64["W"]) >= 0.99
+    assert ari(p32["W"], p64["W"]) >= 0.99
+    for key in ("alpha", "beta", "nu", "delta", "phi", "gamma", "pi"):
+        t32, t64 = p32["traces"][key], p64["traces"][key]
+        se = np.sqrt(mc_se(t32) ** 2 + mc_se(t64) ** 2)
+        d = np.abs(t32.mean(1) - t64.mean(1))
+        live = se > 0
+        frac = np.mean(d[live] <= 3 * se[live])
+        print(f"{key:6s} {live.sum():6d} parameters: {frac:.4f} within 3 MC s.e.")
+        assert frac >= 0.97, key
+        assert np.all(d[~live] == 0), key
