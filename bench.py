@@ -97,14 +97,17 @@ def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
     chains = 4
     probs = [dict(Y=Y, n_b=n_b, K=K, seed=13579, chains=chains, n_iter=4000, n_burnin=2000, n_unchanged=500)
              for K in Ks]
-    sb = bq.Sampler.batch(probs, precision=precision)
+    tr_b, _ = bq.autotune_batch(probs, precision=precision)
+    sb = bq.Sampler.batch(probs, precision=precision, tile_rows=tr_b)
     sb.sweeps(warmup)
     ms_b = sb.time_sweeps(sweeps)
     sb.close()
     total_plain_ms = 0.0
+    tr_p = {}
     for K in Ks:
+        tr_p[K], _ = bq.autotune(Y, n_b, K, seed=13579, precision=precision)
         for c in range(chains):
-            with bq.Sampler(Y, n_b, K, seed=13579, chain0=c, precision=precision) as sp:
+            with bq.Sampler(Y, n_b, K, seed=13579, chain0=c, precision=precision, tile_rows=tr_p[K]) as sp:
                 sp.sweeps(warmup)
                 total_plain_ms += sp.time_sweeps(sweeps)
     n_chains = len(Ks) * chains
@@ -112,9 +115,56 @@ def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
     cs_p = n_chains * sweeps / (total_plain_ms / 1e3)
     N = sum(n_b)
     return {"config": "c4: K=3..10 x 4 chains on c1 data (N=1000, G=3000, B=4)", "chains": n_chains,
+            "tile_rows_batched": tr_b, "tile_rows_plain": tr_p,
             "sweeps": sweeps, "batched_chain_sweeps_per_s": round(cs_b, 1),
             "plain_sequential_chain_sweeps_per_s": round(cs_p, 1), "ratio": round(cs_b / cs_p, 2),
             "batched_cell_gene_updates_per_s": round(cs_b * N * G, 1)}
+
+
+def c5_datasets(n=16):
+    """c5: 16 synthetic data sets, N log-uniform in [500, 30000], G uniform in
+    [1000, 5000], B in {2..6}, K in {3..10}, batch sizes Dir(2) N with at
+    least K + 1 cells per batch; data seed 5000 + index."""
+    out = []
+    for j in range(n):
+        rng = np.random.default_rng(5000 + j)
+        N = int(round(np.exp(rng.uniform(np.log(500), np.log(30000)))))
+        G = int(rng.integers(1000, 5001))
+        B = int(rng.integers(2, 7))
+        K = int(rng.integers(3, 11))
+        N = max(N, B * (K + 1))
+        extra = rng.multinomial(N - B * (K + 1), rng.dirichlet([2.0] * B))
+        out.append(dict(n_b=[int(K + 1 + e) for e in extra], G=G, K=K, seed=5000 + j))
+    return out
+
+
+def heterogeneous_batch(bq, precision, sweeps=20, warmup=3):
+    """c5: the 16 data sets in one batched sampler (one graph launch per sweep)
+    against the same 16 chains run one after another in the plain form."""
+    sets = c5_datasets()
+    probs = []
+    for d in sets:
+        Y = bq.simulate(d["n_b"], d["G"], d["K"], d["seed"])
+        probs.append(dict(Y=Y, n_b=d["n_b"], K=d["K"], seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500))
+    cells = sum(sum(d["n_b"]) * d["G"] for d in sets)
+    tr_b, _ = bq.autotune_batch(probs, precision=precision, sweeps=5)
+    sb = bq.Sampler.batch(probs, precision=precision, tile_rows=tr_b)
+    sb.sweeps(warmup)
+    ms_b = sb.time_sweeps(sweeps)
+    sb.close()
+    ms_p = 0.0
+    for p in probs:
+        tr, _ = bq.autotune(p["Y"], p["n_b"], p["K"], seed=13579, precision=precision, sweeps=5)
+        with bq.Sampler(p["Y"], p["n_b"], p["K"], seed=13579, precision=precision, tile_rows=tr) as sp:
+            sp.sweeps(warmup)
+            ms_p += sp.time_sweeps(sweeps)
+    return {"config": "c5: 16 data sets, N 500-30000 log-uniform, G 1000-5000, B 2-6, K 3-10",
+            "cells_x_genes": cells, "total_cells": sum(sum(d["n_b"]) for d in sets), "sweeps": sweeps,
+            "tile_rows_batched": tr_b,
+            "batched_chain_sweeps_per_s": round(len(sets) * sweeps / (ms_b / 1e3), 1),
+            "plain_sequential_chain_sweeps_per_s": round(len(sets) * sweeps / (ms_p / 1e3), 1),
+            "ratio": round(ms_p / ms_b, 2),
+            "batched_cell_gene_updates_per_s": round(cells * sweeps / (ms_b / 1e3), 1)}
 
 
 def main():
@@ -128,6 +178,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -147,9 +198,11 @@ def main():
     calib_ms = bq.copy_probe(1 << 30)
     log(f"[bench] copy probe 1 GiB: {calib_ms:.3f} ms = {2 * (1 << 30) / calib_ms / 1e6:.0f} GB/s (read+write)")
 
+    tr, _ = bq.autotune(Y, n_b, K, seed=13579, precision=args.precision)
     t0 = time.perf_counter()
-    s = bq.Sampler(Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500, precision=args.precision)
-    log(f"[bench] init {time.perf_counter() - t0:.1f}s")
+    s = bq.Sampler(Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500, precision=args.precision,
+                   tile_rows=tr)
+    log(f"[bench] init {time.perf_counter() - t0:.1f}s, tile rows {tr} (autotuned)")
     s.sweeps(args.warmup)
     w0 = time.perf_counter()
     ms = s.time_sweeps(args.steps)
@@ -180,6 +233,13 @@ def main():
         log(f"[bench] batched c4: {batched['batched_chain_sweeps_per_s']} chain-sweeps/s vs plain sequential "
             f"{batched['plain_sequential_chain_sweeps_per_s']} -> {batched['ratio']}x")
 
+    hetero = None
+    if not args.no_batched and not args.no_c5:
+        hetero = heterogeneous_batch(bq, args.precision)
+        log(f"[bench] batched c5: {hetero['batched_chain_sweeps_per_s']} chain-sweeps/s vs plain sequential "
+            f"{hetero['plain_sequential_chain_sweeps_per_s']} -> {hetero['ratio']}x "
+            f"({hetero['batched_cell_gene_updates_per_s']:.3g} cell x gene updates/s)")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(Y, cfg, args.precision, args.cpu_budget)
@@ -201,7 +261,7 @@ def main():
         "data": "synthetic (BUSseq generative model, seeded)",
         "config": {"workload": f"plain BUSseq Gibbs sweep, one chain, {args.config}: N={N} "
                                f"({'/'.join(map(str, n_b))}), G={G}, K={K}, B={len(n_b)}",
-                   "chains": 1, "precision": args.precision, "parallelism": "single GPU"},
+                   "chains": 1, "precision": args.precision, "tile_rows": tr, "parallelism": "single GPU"},
         "roofline": roof,
         "sweep_roofline": {"bytes_per_sweep": sweep_bytes,
                            "achieved": round(sweep_bytes * sweeps_per_s / 1e9, 2),
@@ -209,6 +269,7 @@ def main():
                            "frac": round(sweep_bytes * sweeps_per_s / 1e9 / HBM_PEAK_GBS, 5)},
         "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
         "batched_vs_plain": batched,
+        "heterogeneous_batch": hetero,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(sweeps_per_s / cpu["value"], 1) if cpu else None,
     }

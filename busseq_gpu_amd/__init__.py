@@ -14,7 +14,7 @@ import os as _os
 import numpy as _np
 
 __all__ = ["Sampler", "simulate", "run", "read_counts", "read_dims", "kernel_names", "copy_probe",
-           "library_path", "BusseqError", "STATUS"]
+           "autotune", "autotune_batch", "library_path", "BusseqError", "STATUS"]
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
 _LIB_PATH = _os.path.join(_HERE, "lib", "libbusseq.so")
@@ -96,6 +96,8 @@ def _load():
     lib.busseq_kernel_name.restype = _C.c_char_p
     lib.busseq_kernel_times.argtypes = [vp, _C.c_int, _PD]
     lib.busseq_copy_probe.argtypes = [_C.c_size_t, _C.c_int, _C.POINTER(_C.c_double)]
+    lib.busseq_autotune_batch.argtypes = [_C.POINTER(_Problem), _C.c_int, _C.c_int, _C.POINTER(_C.c_int),
+                                          _C.POINTER(_C.c_double)]
     lib.busseq_simulate.argtypes = [_C.c_int, _C.POINTER(_C.c_int), _C.c_int, _C.c_int, _C.c_uint32,
                                     _C.c_double, _C.c_double, _PI, _PI, _PI]
     _lib = lib
@@ -223,20 +225,7 @@ class Sampler:
     def _init(self, problems, precision, tile_rows, device):
         lib = _load()
         self._lib, self._h = lib, None
-        keep, probs, dims = [], (_Problem * len(problems))(), []
-        for j, p in enumerate(problems):
-            Y = _np.ascontiguousarray(p["Y"], dtype=_np.int32)
-            N, G = Y.shape
-            n_b = list(p["n_b"])
-            if sum(n_b) != N:
-                raise BusseqError(1, "sum(n_b) must equal the number of rows of Y")
-            cfg, nb = _config(n_b, G, p["K"], p.get("seed", 13579), p.get("n_iter", 4000), p.get("n_burnin", 2000),
-                              p.get("n_unchanged", 500), p.get("keep_traces", False), precision,
-                              p.get("chains", 1), p.get("chain0", 0), tile_rows, device)
-            keep += [Y, cfg, nb]
-            probs[j].cfg = _C.pointer(cfg)
-            probs[j].Y = _ip(Y)
-            dims += [(N, G, int(p["K"]), len(n_b), n_b)] * int(p.get("chains", 1))
+        probs, keep, dims = _problem_array(problems, precision, tile_rows, device)
         h = _C.c_void_p()
         _check(lib.busseq_create_batch(probs, len(problems), _C.byref(h)))
         self._h, self._keep, self._dims = h, keep, dims
@@ -307,6 +296,39 @@ class Sampler:
             return _result_to_dict(r)
         finally:
             self._lib.busseq_result_free(_C.byref(r))
+
+
+def _problem_array(problems, precision, tile_rows, device):
+    keep, probs, dims = [], (_Problem * len(problems))(), []
+    for j, p in enumerate(problems):
+        Y = _np.ascontiguousarray(p["Y"], dtype=_np.int32)
+        N, G = Y.shape
+        n_b = list(p["n_b"])
+        if sum(n_b) != N:
+            raise BusseqError(1, "sum(n_b) must equal the number of rows of Y")
+        cfg, nb = _config(n_b, G, p["K"], p.get("seed", 13579), p.get("n_iter", 4000), p.get("n_burnin", 2000),
+                          p.get("n_unchanged", 500), p.get("keep_traces", False), precision,
+                          p.get("chains", 1), p.get("chain0", 0), tile_rows, device)
+        keep += [Y, cfg, nb]
+        probs[j].cfg = _C.pointer(cfg)
+        probs[j].Y = _ip(Y)
+        dims += [(N, G, int(p["K"]), len(n_b), n_b)] * int(p.get("chains", 1))
+    return probs, keep, dims
+
+
+def autotune_batch(problems, precision=32, sweeps=20, device=0):
+    """Fastest tile size (cells per tile) of {16, 32, 64, 128} for these problems in
+    one batched sampler (busseq_autotune_batch); returns (tile_rows, sweeps_per_s)."""
+    lib = _load()
+    probs, keep, _ = _problem_array(problems, precision, 64, device)
+    tr, sps = _C.c_int(), _C.c_double()
+    _check(lib.busseq_autotune_batch(probs, len(problems), int(sweeps), _C.byref(tr), _C.byref(sps)))
+    return tr.value, sps.value
+
+
+def autotune(Y, n_b, K, seed=13579, chains=1, precision=32, sweeps=20, device=0):
+    """Fastest tile size for one problem (busseq_autotune); returns (tile_rows, sweeps_per_s)."""
+    return autotune_batch([dict(Y=Y, n_b=n_b, K=K, seed=seed, chains=chains)], precision, sweeps, device)
 
 
 def run(Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500, keep_traces=False,

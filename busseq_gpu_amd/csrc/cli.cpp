@@ -6,7 +6,8 @@
 //
 // Additive long options (defaults reproduce the reference's behaviour):
 //   --precision=32|64   per-element arithmetic (default 32)
-//   --tile-rows=R       cells per reduction tile (default 64)
+//   --tile-rows=R|auto  cells per reduction tile (default 64; auto: the fastest of
+//                       16, 32, 64, 128 timed on this data, printed)
 //   --device=D          HIP device ordinal (default 0)
 #include <getopt.h>
 #include <sys/time.h>
@@ -36,7 +37,7 @@ int main(int argc, char** argv)
     char output_file[4096] = "demo_output";
     char n_file[4096] = "";
     int n_file_flag = 0, burnin_flag = 1, unchanged_flag = 1, seed_flag = 1;
-    int precision = 32, tile_rows = 64, device = 0;
+    int precision = 32, tile_rows = 64, device = 0, tune = 0;
 
     static struct option longopts[] = {
         {"precision", required_argument, nullptr, 1000},
@@ -59,7 +60,7 @@ int main(int argc, char** argv)
         case 'p': print_preserved = 1; break;
         case 'o': std::snprintf(output_file, sizeof(output_file), "%s", optarg); break;
         case 1000: precision = std::atoi(optarg); break;
-        case 1001: tile_rows = std::atoi(optarg); break;
+        case 1001: tune = std::strcmp(optarg, "auto") == 0; tile_rows = tune ? 64 : std::atoi(optarg); break;
         case 1002: device = std::atoi(optarg); break;
         default:
             std::printf("Error Usage: %s [-b batch] [-n sample_size_file] [-g gene] [-k celltype] [-c count_data_file)] "
@@ -112,6 +113,12 @@ int main(int argc, char** argv)
     cfg.B = B; cfg.n_b = n_b.data(); cfg.G = G; cfg.K = K;
     cfg.seed = (uint32_t)seed; cfg.n_iter = n_iter; cfg.n_burnin = n_burnin; cfg.n_unchanged = n_unchanged;
     cfg.keep_traces = print_preserved; cfg.precision = precision; cfg.tile_rows = tile_rows; cfg.device = device;
+    if (tune) {
+        double sps = 0.0;
+        if (busseq_autotune(&cfg, Y.data(), 20, &cfg.tile_rows, &sps) != BUSSEQ_OK) return fail("tile autotuning");
+        std::printf("Tile rows %d (autotuned, %.1f sweeps/s).\n", cfg.tile_rows, sps);
+        std::fflush(stdout);
+    }
     busseq_sampler* s = nullptr;
     if (busseq_create(&cfg, Y.data(), &s) != BUSSEQ_OK) return fail("initialization");
 

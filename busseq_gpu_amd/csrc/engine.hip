@@ -1094,6 +1094,7 @@ struct ChainHost {
     int problem;
     std::vector<int> n_b;
     int N, G, Gp, K, B;
+    int T_alloc;                   // tiles the partial-sum buffer holds
 };
 
 struct busseq_sampler {
@@ -1415,7 +1416,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
                 return fail(BUSSEQ_ERR_DEVICE);
             }
             s->hd.push_back(c);
-            s->ch.push_back(ChainHost{p, n_b, N, G, Gp, K, B});
+            s->ch.push_back(ChainHost{p, n_b, N, G, Gp, K, B, T_max});
             busseq_status rc = upload_tcopies(s, (int)s->hd.size() - 1, hi.alpha, hi.beta, hi.nu, hi.delta, hi.phi);
             if (rc != BUSSEQ_OK) return fail(rc);
             auto mx = [](int& a, long v) { if (v > a) a = (int)v; };
@@ -1498,6 +1499,80 @@ extern "C" busseq_status busseq_time_sweeps(busseq_sampler* s, int n, double* ms
     *ms = f;
     s->iter += n;
     return BUSSEQ_OK;
+}
+
+// Tile rows of every chain of a sampler: the tile list k_plan builds, the
+// tile capacity and the tile-pass grid follow; the sweep graph is rebuilt on
+// the next launch.  Only sizes whose tiles fit the partial-sum buffer.
+static busseq_status set_tile_rows(busseq_sampler* s, int tr)
+{
+    if (tr < 1 || tr > BQ_TR_MAX) { bq_set_error("tile_rows must be between 1 and 256"); return BUSSEQ_ERR_ARG; }
+    int maxT = 0;
+    for (int c = 0; c < s->C; ++c) {
+        const ChainHost& h = s->ch[c];
+        const int T = (h.N + tr - 1) / tr + h.B * h.K;
+        if (T > h.T_alloc) { bq_set_error("tile_rows below the size the sampler was created for"); return BUSSEQ_ERR_ARG; }
+        maxT = T > maxT ? T : maxT;
+    }
+    HIPCHK(hipSetDevice(s->dev));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int c = 0; c < s->C; ++c) {
+        s->hd[c].TR = tr;
+        s->hd[c].T_max = (s->ch[c].N + tr - 1) / tr + s->ch[c].B * s->ch[c].K;
+    }
+    s->TR = tr;
+    s->maxT = maxT;
+    HIPCHK(hipMemcpy(s->d_cd, s->hd.data(), sizeof(ChainDev) * s->C, hipMemcpyHostToDevice));
+    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_autotune_batch(const busseq_problem* probs, int n_probs, int sweeps,
+                                               int* tile_rows, double* sweeps_per_s)
+{
+    static const int cand[] = {16, 32, 64, 128};
+    if (!probs || n_probs < 1 || n_probs > 4096 || sweeps < 1 || !tile_rows) {
+        bq_set_error("bad arguments to busseq_autotune");
+        return BUSSEQ_ERR_ARG;
+    }
+    std::vector<busseq_config> cfgs(n_probs);
+    std::vector<busseq_problem> ps(n_probs);
+    for (int p = 0; p < n_probs; ++p) {
+        if (!probs[p].cfg) { bq_set_error("null config"); return BUSSEQ_ERR_ARG; }
+        cfgs[p] = *probs[p].cfg;
+        cfgs[p].tile_rows = cand[0];
+        cfgs[p].keep_traces = 0;
+        ps[p].cfg = &cfgs[p];
+        ps[p].Y = probs[p].Y;
+    }
+    busseq_sampler* s = nullptr;
+    busseq_status rc = busseq_create_batch(ps.data(), n_probs, &s);
+    if (rc != BUSSEQ_OK) return rc;
+    int best = 64;
+    double best_sps = -1.0;
+    for (int tr : cand) {
+        double ms = 0.0;
+        rc = set_tile_rows(s, tr);
+        if (rc == BUSSEQ_OK) rc = busseq_sweeps(s, 2);
+        if (rc == BUSSEQ_OK) rc = busseq_time_sweeps(s, sweeps, &ms);
+        if (rc != BUSSEQ_OK) { busseq_destroy(s); return rc; }
+        const double sps = sweeps / (ms * 1e-3);
+        if (sps > best_sps) { best_sps = sps; best = tr; }
+    }
+    busseq_destroy(s);
+    *tile_rows = best;
+    if (sweeps_per_s) *sweeps_per_s = best_sps;
+    return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_autotune(const busseq_config* cfg, const int32_t* Y, int sweeps,
+                                         int* tile_rows, double* sweeps_per_s)
+{
+    busseq_problem p;
+    p.cfg = cfg;
+    p.Y = Y;
+    return busseq_autotune_batch(&p, 1, sweeps, tile_rows, sweeps_per_s);
 }
 
 extern "C" int busseq_kernel_count(void) { return kNumKernels; }

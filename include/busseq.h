@@ -121,12 +121,28 @@ int busseq_chain_count(const busseq_sampler* s);
 busseq_status busseq_read_dims(const char* path, int B, int* n_b);
 busseq_status busseq_read_counts(const char* path, int N, int G, int32_t* Y);
 
-/* Synthetic counts from the BUSseq generative model (see DESIGN of the
- * benchmark configurations).  gamma0 ~ U(g0_lo, g0_hi), gamma1 ~ U(-0.3,-0.1).
+/* Synthetic counts from the BUSseq generative model (no counterpart in the
+ * reference): alpha_g ~ N(1.5, 0.5^2); beta_gk = 0 for k = 0 and N(+-2, 0.5^2)
+ * on 10% of the genes for each k >= 1; nu_bg = 0 for b = 0, else N(0, 0.5^2);
+ * delta_i ~ N(0, 0.2^2), 0 for each batch's first cell; phi_bg ~ Gamma(5, rate
+ * 0.5); pi_b ~ Dirichlet(1); X ~ NB(exp(alpha + beta + nu + delta), phi);
+ * gamma0 ~ U(g0_lo, g0_hi), gamma1 ~ U(-0.3, -0.1); Z ~ Bernoulli(logistic(
+ * gamma0 + gamma1 X)); Y = X (1 - Z).  Streams keyed by data_seed.
  * Outputs that are NULL are skipped: Y (N*G), true W (N), X (N*G). */
 busseq_status busseq_simulate(int B, const int* n_b, int G, int K, uint32_t data_seed,
                               double g0_lo, double g0_hi, int32_t* Y, int32_t* W_true,
                               int32_t* X_true);
+
+/* Tile autotuner (no counterpart in the reference): times `sweeps` sweeps of
+ * the given problem(s) with each tile size in {16, 32, 64, 128} cells and
+ * returns the fastest in *tile_rows (and its sweeps/s).  The tile size sets
+ * the grouping of the per-gene sums (their last-bit rounding) and not the
+ * model; a run is reproducible given the tile_rows it used.  The problems are
+ * not modified; the timing chains are discarded. */
+busseq_status busseq_autotune(const busseq_config* cfg, const int32_t* Y, int sweeps,
+                              int* tile_rows, double* sweeps_per_s);
+busseq_status busseq_autotune_batch(const busseq_problem* problems, int n_problems, int sweeps,
+                                    int* tile_rows, double* sweeps_per_s);
 
 /* Measurement helpers (no counterpart in the reference, which has no timers).
  * busseq_kernel_times runs n sweeps as individual launches with HIP events

@@ -64,3 +64,20 @@ def test_k_sweep_bic_per_k():
         with bq.Sampler(Y, n_b, Ks[q], seed=5, chain0=1, n_iter=40, n_burnin=20, n_unchanged=5) as sp:
             sp.sweeps(40)
             assert sp.finish()["bic"] == bic[q][1]
+
+
+def test_autotune_returns_a_candidate_tile_size_that_the_oracle_reproduces():
+    from oracle import pyoracle as po
+    n_b, G, K = [40, 30, 30], 150, 3
+    Y = bq.simulate(n_b, G, K, 21)
+    tr, sps = bq.autotune(Y, n_b, K, seed=9, sweeps=5)
+    assert tr in (16, 32, 64, 128) and sps > 0
+    tr_b, _ = bq.autotune_batch(problems(), sweeps=3)
+    assert tr_b in (16, 32, 64, 128)
+    # a chain run with the chosen tile size equals the oracle with the same grouping
+    with bq.Sampler(Y, n_b, K, seed=9, tile_rows=tr, n_iter=20, n_burnin=10, n_unchanged=3) as s:
+        s.sweeps(6)
+        g = s.state()
+    o = po.Oracle(Y, n_b, K, seed=9, n_iter=20, n_burnin=10, n_unchanged=3, tile_rows=tr)
+    o.sweeps(6)
+    same(g, o.state(), f"tile rows {tr}")
