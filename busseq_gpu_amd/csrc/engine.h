@@ -34,6 +34,7 @@
     const Q int64_t* hist_s;  /* [B*H] static histogram of Y>0 entries with Y<H */ \
     const Q int32_t* zrow;  /* [N+1] CSR row offsets of the Y = 0 entries */ \
     const Q uint16_t* zcol;  /* [nz]  their genes */ \
+    Q int32_t* zs;  /* [nz]  their current S values, kept equal to S at those entries */ \
     const Q int32_t* big_off;  /* [B+1] static counts Y >= H, per batch */ \
     const Q int32_t* big;  /* [nbig] their values */ \
     /* state */ \

@@ -328,6 +328,7 @@ struct XzItem {
     int32_t s;
     uint32_t b0, b1, b2;   // unread words of the entry's Philox block after the Z draw
     double mu;
+    double phi;
 };
 
 __device__ __forceinline__ void wave_lds_sync()
@@ -350,6 +351,7 @@ __device__ __forceinline__ void xz_big_terms(double g0, double g1, double g0p, d
 }
 
 static_assert(BQ_HLO == 64, "k_xz keeps one low histogram bin per lane");
+#define XZ_SPT 256   // softplus table entries staged in LDS (every inversion draw is below it)
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
@@ -357,8 +359,18 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ uint32_t hist[4][BQ_HLO];
     __shared__ XzItem queue[4][128];
+    __shared__ double s_spt[XZ_SPT];
+    __shared__ double s_inv[BQ_INV_N];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 4 + wv;
+    // the grid covers the largest chain of a batch: blocks past this chain's cells leave
+    if (blockIdx.x * 4 >= c.N) return;
+    // the softplus table of the batch of the block's first cell; a wave whose cell
+    // lies in another batch reads its own batch's table from memory
+    const int bb = c.batch[blockIdx.x * 4];
+    for (int x = threadIdx.x; x < XZ_SPT; x += 256) s_spt[x] = c.sp_tab[(size_t)bb * H + x];
+    for (int x = threadIdx.x; x < BQ_INV_N; x += 256) s_inv[x] = bq_inv_tab[x];
+    __syncthreads();
     if (i >= c.N) return;
     uint32_t* hw = hist[wv];
     XzItem* qw = queue[wv];
@@ -373,26 +385,36 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
     const T* en = (const T*)c.en + (size_t)b * G;
     auto phi = c.phi + (size_t)b * G;
     auto spt = c.sp_tab + (size_t)b * H;
+    const bool own = b == bb;
     auto row = c.S + (size_t)i * c.Gp;
+    auto zs = c.zs;
     const int z0 = c.zrow[i], z1 = c.zrow[i + 1];
     const uint64_t below = (1ull << lane) - 1ull;
     uint64_t n1 = 0, sx1 = 0;
     int qn = 0, nt = 0;
     const int stripe = blockIdx.x % BQ_STRIPES;
     auto tl = c.xz_tail + z0;
+    // entries of the current round; the next round's are loaded while this one runs
+    int gc = 0;
+    int32_t sc = 0;
+    if (z0 + lane < z1) { gc = c.zcol[z0 + lane]; sc = zs[z0 + lane]; }
     for (int base = z0;; base += 64) {
         const bool more = base < z1;
+        int gn = 0;
+        int32_t sn_next = 0;
         if (more) {
+            const int pn = base + 64 + lane;
+            if (pn < z1) { gn = c.zcol[pn]; sn_next = zs[pn]; }
             const int p = base + lane;
             const bool act = p < z1;
-            int g = 0, z = 0;
-            int32_t s = 0;
+            int z = 0;
             bq_rng r;
+            T e_a = (T)0, e_b = (T)0, e_n = (T)0;
+            double ph = 0.0;
             if (act) {
-                g = c.zcol[p];
-                s = row[g];
-                bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ);
-                const int xo = s < 0 ? ~s : 0;   // X = 0 redraws Z whatever Z was (cu:493-496)
+                e_a = ea[gc]; e_b = eb[gc]; e_n = en[gc]; ph = phi[gc];
+                bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)gc, (uint32_t)t, BQ_PH_XZ);
+                const int xo = sc < 0 ? ~sc : 0;   // X = 0 redraws Z whatever Z was (cu:493-496)
                 z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
             }
             const uint64_t m = __ballot(act && z);
@@ -400,13 +422,14 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
             if (act && z) {
                 XzItem it;
                 it.p = p;
-                it.g = g;
-                it.s = s;
+                it.g = gc;
+                it.s = sc;
                 it.b0 = r.b0; it.b1 = r.b1; it.b2 = r.b2;
-                it.mu = (double)tmu(ea[g], eb[g], en[g], edi);
+                it.mu = (double)tmu(e_a, e_b, e_n, edi);
+                it.phi = ph;
                 qw[qn + __popcll(m & below)] = it;
             }
-            if (act && !z && s != 0) row[g] = 0;
+            if (act && !z && sc != 0) { row[gc] = 0; zs[p] = 0; }
             if (lane == 0 && m0) atomicAdd(&hw[0], (uint32_t)__popcll(m0));
             qn += __popcll(m);
         }
@@ -426,13 +449,14 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
                 r.c3 = ((uint32_t)BQ_PH_XZ << 20) + (xo == 0 ? 1u : 0u);
                 r.b0 = it.b0; r.b1 = it.b1; r.b2 = it.b2; r.b3 = 0u; r.n = xo == 0 ? 3 : 0;
                 const double u = bq_uniform(&r);
-                const double ph = phi[g];
-                if (xo < H && bq_nb_inv_ok(it.mu, ph)) {
-                    // the inversion returns x' <= 255 < H: both softplus terms are table entries
-                    const int xp = bq_rnbinom_inv(u, it.mu, ph);
-                    const int xi = bq_log(bq_uniform(&r)) <= spt[xo] - spt[xp] ? xp : xo;
+                if (xo < XZ_SPT && bq_nb_inv_ok(it.mu, it.phi)) {
+                    // the inversion returns x' <= 255: both softplus terms are table entries
+                    const int xp = bq_rnbinom_inv_t(u, it.mu, it.phi, s_inv);
+                    const double lo = own ? s_spt[xo] : spt[xo];
+                    const double lp = own ? s_spt[xp] : spt[xp];
+                    const int xi = bq_log(bq_uniform(&r)) <= lo - lp ? xp : xo;
                     const int32_t sn = ~xi;
-                    if (sn != it.s) row[g] = sn;
+                    if (sn != it.s) { row[g] = sn; zs[it.p] = sn; }
                     n1 += 1;
                     sx1 += (uint64_t)xi;
                     if (xi < BQ_HLO) atomicAdd(&hw[xi], 1u);
@@ -449,6 +473,8 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
             wave_lds_sync();
         }
         if (!more) break;
+        gc = gn;
+        sc = sn_next;
     }
     n1 = wave_sum_u64(n1);
     sx1 = wave_sum_u64(sx1);
@@ -507,7 +533,8 @@ __global__ __launch_bounds__(256) void k_xz_tail(const ChainDev* __restrict__ cd
         for (int q = 0; q < TAIL_CPW - 1; ++q) m += incl[wv][q] <= j;
         const int i = i0 + m;
         const int l = j - (m ? incl[wv][m - 1] : 0);
-        const int g = c.zcol[c.xz_tail[c.zrow[i] + l]];
+        const int pz = c.xz_tail[c.zrow[i] + l];
+        const int g = c.zcol[pz];
         const int b = c.batch[i], k = c.W[i];
         auto e = c.S + (size_t)i * c.Gp + g;
         const int32_t s = *e;
@@ -518,7 +545,7 @@ __global__ __launch_bounds__(256) void k_xz_tail(const ChainDev* __restrict__ cd
         bq_rng r;
         bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)g, (uint32_t)t, BQ_PH_XZ);
         const int32_t sn = bq_xz_update(s, (double)mu, c.phi[(size_t)b * G + g], g0, g1, qz, &r);
-        if (sn != s) *e = sn;
+        if (sn != s) { *e = sn; c.zs[pz] = sn; }
         const int xi = sn < 0 ? ~sn : 0;
         const int bl = b - bmin;
         if (sn < 0) {
@@ -1384,6 +1411,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.hist_lo = s->dalloc<uint32_t>((size_t)B * BQ_HLO * BQ_STRIPES);
             c.gstat = s->dalloc<uint64_t>((size_t)B * 5 * BQ_STRIPES);
             c.xz_tail = s->dalloc<int32_t>(nz > 0 ? (size_t)nz : 1);
+            c.zs = s->dalloc<int32_t>(nz > 0 ? (size_t)nz : 1);   // every Y = 0 entry starts at X = 0, Z = 0
             c.xz_ntail = s->dalloc<int32_t>(N);
             c.sp_tab = s->dalloc<double>((size_t)B * H);
             c.beta_acc = s->dalloc<int32_t>((size_t)K * G);
@@ -1689,6 +1717,18 @@ extern "C" busseq_status busseq_set_state(busseq_sampler* s, int ch, const busse
                 v = z ? ~x : 0;
             }
         HIPCHK(hipMemcpy(c.S, S.data(), S.size() * 4, hipMemcpyHostToDevice));
+        // the compact copy of the Y = 0 entries follows S
+        std::vector<int32_t> zrow(N + 1);
+        HIPCHK(hipMemcpy(zrow.data(), c.zrow, 4 * ((size_t)N + 1), hipMemcpyDeviceToHost));
+        const size_t nz = (size_t)zrow[N];
+        if (nz > 0) {
+            std::vector<uint16_t> zcol(nz);
+            std::vector<int32_t> zs(nz);
+            HIPCHK(hipMemcpy(zcol.data(), c.zcol, 2 * nz, hipMemcpyDeviceToHost));
+            for (int i = 0; i < N; ++i)
+                for (int q = zrow[i]; q < zrow[i + 1]; ++q) zs[q] = S[(size_t)i * Gp + zcol[q]];
+            HIPCHK(hipMemcpy(c.zs, zs.data(), 4 * nz, hipMemcpyHostToDevice));
+        }
     }
     auto put = [&](void* dst, const void* src, size_t n) -> hipError_t {
         return src ? hipMemcpy(dst, src, n, hipMemcpyHostToDevice) : hipSuccess;

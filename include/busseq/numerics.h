@@ -561,7 +561,7 @@ BQ_FN int bq_rnbinom(bq_rng* r, double mu, double phi)
  */
 BQ_FN int bq_nb_inv_ok(double mu, double phi) { return mu <= 30.0 && 3.0 * mu <= 17.0 * phi; }
 
-BQ_FN int bq_rnbinom_inv(double u, double mu, double phi)
+BQ_FN int bq_rnbinom_inv_t(double u, double mu, double phi, const double* inv)
 {
     double q = mu / (phi + mu);
     double p = bq_exp(phi * bq_log1p(-q));
@@ -569,11 +569,15 @@ BQ_FN int bq_rnbinom_inv(double u, double mu, double phi)
     int k = 0;
     while (u > s && k < 255) {
         k += 1;
-        p *= q * ((phi + (double)(k - 1)) * bq_inv_tab[BQ_WAVE_UNIFORM(k)]);
+        p *= q * ((phi + (double)(k - 1)) * inv[BQ_WAVE_UNIFORM(k)]);
         s += p;
     }
     return k;
 }
+
+/* inv: the reciprocals 1/k of bq_inv_tab, or a copy of them (device code may
+ * pass one staged in LDS) */
+BQ_FN int bq_rnbinom_inv(double u, double mu, double phi) { return bq_rnbinom_inv_t(u, mu, phi, bq_inv_tab); }
 
 /* The proposal draw of the X/Z step: the first uniform of the stream drives
  * the inversion where bq_nb_inv_ok holds; elsewhere it is skipped and the
