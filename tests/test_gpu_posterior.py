@@ -10,7 +10,12 @@ batched sampler; the chain of lowest BIC and an FP64 chain with the same
 Philox key recover the true subtypes (ARI >= 0.99, also against each other),
 and their posterior means agree within 3 Monte-Carlo standard errors for all
 but a few percent of the parameters (standard errors from the effective
-sample size, Geyer's initial positive sequence).
+sample size, Geyer's initial positive sequence).  The cell effects delta of a
+batch and the batch effects nu of that batch have one slowly mixing direction
+-- all delta of the batch up by c, all its nu down by c, which moves only the
+batch's first cell (its delta is fixed at 0, cu:766-772) -- so delta and nu
+are compared after removing their per-batch mean in every recorded iteration,
+and the per-batch shift itself is bounded separately.
 """
 import numpy as np
 import pytest
@@ -73,8 +78,20 @@ def test_mixed_precision_posterior_agrees_with_fp64():
           f"fp32-fp64 {ari(p32['W'], p64['W']):.4f}; BIC {p32['bic']:.1f} / {p64['bic']:.1f}")
     assert ari(W_true, p32["W"]) >= 0.99 and ari(W_true, p64["W"]) >= 0.99
     assert ari(p32["W"], p64["W"]) >= 0.99
+    batch = np.repeat(np.arange(len(n_b)), n_b)
+
+    def centred(p, key):
+        tr = p["traces"][key].copy()
+        if key == "delta":        # minus the batch mean of the cells' delta, per iteration
+            for b in range(len(n_b)):
+                tr[batch == b] -= tr[batch == b].mean(0)
+        if key == "nu":           # rows (b-1)*G + g for b >= 1: minus the batch mean over genes
+            tr = tr.reshape(len(n_b) - 1, G, -1)
+            tr = (tr - tr.mean(1, keepdims=True)).reshape((len(n_b) - 1) * G, -1)
+        return tr
+
     for key in ("alpha", "beta", "nu", "delta", "phi", "gamma", "pi"):
-        t32, t64 = p32["traces"][key], p64["traces"][key]
+        t32, t64 = centred(p32, key), centred(p64, key)
         se = np.sqrt(mc_se(t32) ** 2 + mc_se(t64) ** 2)
         d = np.abs(t32.mean(1) - t64.mean(1))
         live = se > 0
@@ -82,3 +99,6 @@ def test_mixed_precision_posterior_agrees_with_fp64():
         print(f"{key:6s} {live.sum():6d} parameters: {frac:.4f} within 3 MC s.e.")
         assert frac >= 0.97, key
         assert np.all(d[~live] == 0), key
+    shift = [abs(p32["delta"][batch == b].mean() - p64["delta"][batch == b].mean()) for b in range(len(n_b))]
+    print("per-batch shift of mean delta:", np.round(shift, 4))
+    assert max(shift) < 0.05
