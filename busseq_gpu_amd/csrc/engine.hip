@@ -572,6 +572,38 @@ __global__ __launch_bounds__(256) void k_xz_tail(const ChainDev* __restrict__ cd
 }
 
 // ---------------------------------------------------------------------------
+// The cells of a tile, four at a time: their S entries at gene g and their
+// precision-T e^delta are loaded together (four loads in flight), then f(s, e)
+// runs for each cell in tile order, so every sum keeps its order.
+// ---------------------------------------------------------------------------
+template <typename T, typename F>
+__device__ __forceinline__ void for_tile_cells(const ChainG& c, int st, int nr, int g, F&& f)
+{
+    const T* ed = (const T*)c.ed;
+    const int32_t* S = (const int32_t*)c.S;
+    const size_t Gp = c.Gp;
+    int rr = 0;
+    for (; rr + 4 <= nr; rr += 4) {
+        int ii[4];
+        int32_t ss[4];
+        T ee[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ii[u] = c.perm[st + rr + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            ss[u] = S[(size_t)ii[u] * Gp + g];
+            ee[u] = ed[ii[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) f(ss[u], ee[u]);
+    }
+    for (; rr < nr; ++rr) {
+        const int i = c.perm[st + rr];
+        f(S[(size_t)i * Gp + g], ed[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_alpha: alpha partial sums (cu:608-623).  Block = 256 genes x one tile.
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -589,16 +621,13 @@ __global__ __launch_bounds__(256) void k_alpha(const ChainDev* __restrict__ cds)
     const T eag = ((const T*)c.ea)[g], ebg = ((const T*)c.eb)[(size_t)k * G + g];
     const T eng = ((const T*)c.en)[bg], phg = ((const T*)c.ph)[bg];
     const T dag = ((const T*)c.da)[g], cag = ((const T*)c.ca)[g];
-    const T* ed = (const T*)c.ed;
     double acc = 0.0;
-    for (int rr = 0; rr < nr; ++rr) {
-        const int i = c.perm[st + rr];
-        const int32_t s = c.S[(size_t)i * c.Gp + g];
+    for_tile_cells<T>(c, st, nr, g, [&](int32_t s, T edi) {
         const int xi = s < 0 ? ~s : s;
-        const T mu = tmu(eag, ebg, eng, ed[i]);
+        const T mu = tmu(eag, ebg, eng, edi);
         const T rt = tratio(mu, phg);
         acc += (double)ttmh((T)xi, phg, rt, dag, cag);
-    }
+    });
     c.part[((size_t)tile * 3) * c.Gp + g] = bq_fix(acc);
 }
 
@@ -815,14 +844,11 @@ __global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cd
     const T dbg = ((const T*)c.db)[kg], cbg = ((const T*)c.cb)[kg];
     const T dng = ((const T*)c.dn)[bg], cng = ((const T*)c.cn)[bg];
     const T cbn = (cbg + cng) + cbg * cng;
-    const T* ed = (const T*)c.ed;
     double pb = 0.0, pn0 = 0.0, pn1 = 0.0;
-    for (int rr = 0; rr < nr; ++rr) {
-        const int i = c.perm[st + rr];
-        const int32_t s = c.S[(size_t)i * c.Gp + g];
+    for_tile_cells<T>(c, st, nr, g, [&](int32_t s, T edi) {
         const int xi = s < 0 ? ~s : s;
         const T x = (T)xi;
-        const T mu = tmu(eag, ebg, eng, ed[i]);
+        const T mu = tmu(eag, ebg, eng, edi);
         const T rt = tratio(mu, phg);
         T lb = (T)0;
         if (k >= 1) {
@@ -833,7 +859,7 @@ __global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cd
             pn0 += (double)ttmh(x, phg, rt, dng, cng);
             if (k >= 1) pn1 += (double)ttmh2(x, phg, rt, dng, cbn, lb);
         }
-    }
+    });
     auto pt = c.part + (size_t)tile * 3 * c.Gp;
     pt[g] = bq_fix(pb);
     pt[c.Gp + g] = bq_fix(pn0);
@@ -931,13 +957,24 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
     for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
         const int c1 = c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G;
         double part = 0.0;
-        for (int g = c0 + lane; g < c1; g += 64) {
-            int32_t s = row[g];
-            int xi = s < 0 ? ~s : s;
-            T phi = ph[g];
-            T mu = tmu(ea[g], eb[g], en[g], edi);
-            T rt = tratio(mu, phi);
-            part += (double)ttmh((T)xi, phi, rt, dd, cd);
+        // four genes of the lane class per step, loaded together, added in gene order
+        for (int g0 = c0 + lane; g0 < c1; g0 += 256) {
+            int32_t sv[4];
+            T pv[4], av[4], bv[4], nv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int g = g0 + 64 * u;
+                if (g < c1) { sv[u] = row[g]; pv[u] = ph[g]; av[u] = ea[g]; bv[u] = eb[g]; nv[u] = en[g]; }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (g0 + 64 * u < c1) {
+                    const int xi = sv[u] < 0 ? ~sv[u] : sv[u];
+                    const T mu = tmu(av[u], bv[u], nv[u], edi);
+                    const T rt = tratio(mu, pv[u]);
+                    part += (double)ttmh((T)xi, pv[u], rt, dd, cd);
+                }
+            }
         }
         a += bq_fix(part);
     }
@@ -972,15 +1009,12 @@ __global__ __launch_bounds__(256) void k_phi(const ChainDev* __restrict__ cds)
     const T eng = ((const T*)c.en)[bg], phg = ((const T*)c.ph)[bg];
     const T fp = pc[bg], d = pc[BG + bg], dop = pc[2 * BG + bg], ip = pc[3 * BG + bg];
     const T lrat = pc[4 * BG + bg], dlg0 = pc[5 * BG + bg];
-    const T* ed = (const T*)c.ed;
     double acc = 0.0;
-    for (int rr = 0; rr < nr; ++rr) {
-        const int i = c.perm[st + rr];
-        const int32_t s = c.S[(size_t)i * c.Gp + g];
+    for_tile_cells<T>(c, st, nr, g, [&](int32_t s, T edi) {
         const int xi = s < 0 ? ~s : s;
-        const T mu = tmu(eag, ebg, eng, ed[i]);
+        const T mu = tmu(eag, ebg, eng, edi);
         acc += (double)ttphi(xi, (T)xi, phg, fp, mu, d, dop, ip, lrat, dlg0);
-    }
+    });
     c.part[((size_t)tile * 3) * c.Gp + g] = bq_fix(acc);
 }
 

@@ -410,7 +410,7 @@ BQ_FN float bq_lgamma_diff32(float z, float d)
     float iz = zp * w, izp = z * w;
     float core = d * bq_logf_norm(zp) + ((z - 0.5f) * bq_log1pf(d * iz) - d);
     float dl = core + (bq_stirling_s32(izp) - bq_stirling_s32(iz));
-    if (shifted) dl = dl - (bq_logf(pzp) - bq_logf(pz));
+    if (shifted) dl = dl - bq_logf(pzp / pz);
     return dl;
 }
 
