@@ -94,6 +94,20 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh)
 template <typename V>
 __device__ __forceinline__ V gatomic_add(BQ_GLOBAL V* p, V v) { return atomicAdd((V*)p, v); }
 
+// sum of the BQ_STRIPES copies p[r * stride] of a striped counter, every load
+// issued before the first add (integer adds: exact in any order)
+template <typename V, typename P>
+__device__ __forceinline__ V sum_stripes(const P* p, size_t stride)
+{
+    V v[BQ_STRIPES];
+#pragma unroll
+    for (int r = 0; r < BQ_STRIPES; ++r) v[r] = (V)p[(size_t)r * stride];
+    V acc = 0;
+#pragma unroll
+    for (int r = 0; r < BQ_STRIPES; ++r) acc += v[r];
+    return acc;
+}
+
 __device__ __forceinline__ double ph_uniform(const ChainG& c, uint32_t a, uint32_t b, int t, int phase)
 {
     bq_rng r;
@@ -238,9 +252,7 @@ __global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
     __shared__ int s_cnt[BQ_BMAX * BQ_KMAX];
     __shared__ int s_pos[256 * BQ_KMAX];
     for (int j = tid; j < B * K; j += 256) {
-        int n = 0;
-        for (int r = 0; r < BQ_STRIPES; ++r) n += c.cnt[(size_t)r * B * K + j];
-        s_cnt[j] = n;
+        s_cnt[j] = sum_stripes<int>((const int32_t*)c.cnt + j, (size_t)B * K);
     }
     __syncthreads();
     if (tid == 0) {
@@ -762,16 +774,14 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
     for (int x = tid; x < H; x += 256) {
         int64_t n = hs[x];
         if (x < BQ_HLO) {
-            for (int r = 0; r < BQ_STRIPES; ++r) n += (int64_t)c.hist_lo[(size_t)r * B * BQ_HLO + b * BQ_HLO + x];
+            n += sum_stripes<int64_t>((const uint32_t*)c.hist_lo + (size_t)b * BQ_HLO + x, (size_t)B * BQ_HLO);
         } else {
             n += (int64_t)c.hist_d[(size_t)b * H + x];
         }
         s_n[x] = n;
     }
     if (tid < 5) {
-        uint64_t v = 0;
-        for (int r = 0; r < BQ_STRIPES; ++r) v += c.gstat[(size_t)r * B * 5 + b * 5 + tid];
-        s_gs[tid] = v;
+        s_gs[tid] = sum_stripes<uint64_t>((const uint64_t*)c.gstat + (size_t)b * 5 + tid, (size_t)B * 5);
     }
     __syncthreads();
     const double g0 = c.gamma[b], g1 = c.gamma[B + b];
