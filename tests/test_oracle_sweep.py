@@ -238,13 +238,13 @@ def test_tile_rows_change_only_rounding():
 @pytest.mark.slow
 def test_recovers_synthetic_truth():
     # A single short chain can settle in a mode with two subtypes merged or one
-    # emptied (the posterior is multimodal); the chain of lowest BIC among four
+    # emptied (the posterior is multimodal); the chain of lowest BIC among eight
     # recovers the truth, and every chain that missed it has a higher BIC.
     from sklearn.metrics import adjusted_rand_score
     n_b = [60, 50, 40]
     Y, W, X = bq.simulate(n_b, 300, 3, 2024, want_truth=True)
     runs = []
-    for chain in range(4):
+    for chain in range(8):
         o = po.Oracle(Y, n_b, 3, seed=13579, chain=chain, precision=32, n_iter=300, n_burnin=150,
                       n_unchanged=50)
         o.sweeps(300)
