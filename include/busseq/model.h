@@ -74,22 +74,22 @@ enum {
 /* ------------------------------------------------------------------------ */
 /* per-element terms                                                          */
 /* ------------------------------------------------------------------------ */
-#define BQ_DEFINE_ELEMENT_TERMS(T, SFX, LOG1P, LGDIFF)                              \
+#define BQ_DEFINE_ELEMENT_TERMS(T, SFX, LOG1P, LGDIFF, FMA)                         \
 /* mean: ((e^alpha * e^beta) * e^nu) * e^delta */                                  \
 BQ_FN T bq_mu_##SFX(T ea, T eb, T en, T ed) { return ((ea * eb) * en) * ed; }     \
 /* r = mu / (phi + mu) */                                                          \
 BQ_FN T bq_ratio_##SFX(T mu, T phi) { return mu / (phi + mu); }                    \
 /* subtype-k logit relative to subtype 0: x*beta_k - (phi+x)*log1p(r0*expm1(beta_k)) */ \
 BQ_FN T bq_tw_##SFX(T x, T phi, T r0, T beta, T em)                                \
-{ return x * beta - (phi + x) * LOG1P(r0 * em); }                                  \
+{ return FMA(-(phi + x), LOG1P(r0 * em), x * beta); }                             \
 /* random-walk MH term for a log-mean shift dlt, c = expm1(dlt); l = log1p(r*c) */ \
-BQ_FN T bq_tmh_l_##SFX(T x, T phi, T dlt, T l) { return x * dlt - (phi + x) * l; } \
+BQ_FN T bq_tmh_l_##SFX(T x, T phi, T dlt, T l) { return FMA(-(phi + x), l, x * dlt); } \
 BQ_FN T bq_tmh_##SFX(T x, T phi, T r, T dlt, T c)                                  \
 { return bq_tmh_l_##SFX(x, phi, dlt, LOG1P(r * c)); }                              \
 /* shift dlt applied on top of an accepted shift whose log1p term is l1:           \
  * c12 = expm1(dlt1 + dlt) */                                                       \
 BQ_FN T bq_tmh2_##SFX(T x, T phi, T r, T dlt, T c12, T l1)                         \
-{ return x * dlt - (phi + x) * (LOG1P(r * c12) - l1); }                            \
+{ return FMA(-(phi + x), LOG1P(r * c12) - l1, x * dlt); }                         \
 /* phi MH term per cell (BUSseq_gpu.cu:799-802) rewritten as                        \
  *   [lgamma(phi'+x)-lgamma(phi+x)] - [lgamma(phi')-lgamma(phi)] - x*log(phi'/phi)  \
  *   + (phi+x)*log1p(d*mu/(phi*(phi'+mu))) - d*log1p(mu/phi'),  d = phi'-phi.       \
@@ -103,8 +103,8 @@ BQ_FN T bq_tphi_##SFX(int xi, T x, T phi, T phip, T mu, T d, T dop, T iphip, T l
     return t;                                                                      \
 }
 
-BQ_DEFINE_ELEMENT_TERMS(float, f32, bq_log1pf, bq_lgamma_diff32)
-BQ_DEFINE_ELEMENT_TERMS(double, f64, bq_log1p, bq_lgamma_diff64)
+BQ_DEFINE_ELEMENT_TERMS(float, f32, bq_log1pf, bq_lgamma_diff32, __builtin_fmaf)
+BQ_DEFINE_ELEMENT_TERMS(double, f64, bq_log1p, bq_lgamma_diff64, __builtin_fma)
 
 /* ------------------------------------------------------------------------ */
 /* per-parameter MH terms (double precision)                                  */

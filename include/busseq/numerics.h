@@ -187,9 +187,9 @@ BQ_FN double bq_log(double x)
     int e;
     double m = bq_split64(x, &e);
     double f = m - 1.0;
-    double r = f + (f * f) * bq_log1p_q64(f);
+    double r = __builtin_fma(f * f, bq_log1p_q64(f), f);
     double ed = (double)(e + eadj);
-    return ed * BQ_LN2_HI + (ed * BQ_LN2_LO + r);
+    return __builtin_fma(ed, BQ_LN2_HI, __builtin_fma(ed, BQ_LN2_LO, r));
 }
 
 BQ_FN double bq_log1p(double y)
@@ -202,9 +202,9 @@ BQ_FN double bq_log1p(double y)
         double m = bq_split64(1.0 + y, &e);
         f = m - 1.0;
     }
-    double r = f + (f * f) * bq_log1p_q64(f);
+    double r = __builtin_fma(f * f, bq_log1p_q64(f), f);
     double ed = (double)e;
-    return ed * BQ_LN2_HI + (ed * BQ_LN2_LO + r);
+    return __builtin_fma(ed, BQ_LN2_HI, __builtin_fma(ed, BQ_LN2_LO, r));
 }
 
 /* (exp(r) - 1 - r) / r^2 on |r| <= ln2/2, degree 11, rel err 6e-19 */
@@ -331,9 +331,9 @@ BQ_FN float bq_logf(float x)
     int e;
     float m = bq_split32(xs, &e);
     float f = m - 1.0f;
-    float r = f + (f * f) * bq_log1p_q32(f);
+    float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
     float ef = (float)(e + (sub ? -25 : 0));
-    float res = ef * BQ_LN2_HI_F + (ef * BQ_LN2_LO_F + r);
+    float res = __builtin_fmaf(ef, BQ_LN2_HI_F, __builtin_fmaf(ef, BQ_LN2_LO_F, r));
     if (!(x > 0.0f)) res = x == 0.0f ? -BQ_INFF : BQ_NANF;
     if (x == BQ_INFF) res = BQ_INFF;
     return res;
@@ -347,9 +347,9 @@ BQ_FN float bq_logf_norm(float x)
     int e;
     float m = bq_split32(x, &e);
     float f = m - 1.0f;
-    float r = f + (f * f) * bq_log1p_q32(f);
+    float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
     float ef = (float)e;
-    return ef * BQ_LN2_HI_F + (ef * BQ_LN2_LO_F + r);
+    return __builtin_fmaf(ef, BQ_LN2_HI_F, __builtin_fmaf(ef, BQ_LN2_LO_F, r));
 }
 
 BQ_FN float bq_log1pf_rare(float y)
@@ -374,9 +374,10 @@ BQ_FN float bq_log1pf(float y)
     float s = bq_bitsf(0x3F800000u - kb);                        /* 2^-k */
     float f = __builtin_fmaf(y, s, s - 1.0f);
     float ef = (float)((int32_t)kb >> 23);
-    float r = f + (f * f) * bq_log1p_q32(f);
-    float res = ef * BQ_LN2_HI_F + (ef * BQ_LN2_LO_F + r);
-    if (!(y > -1.0f && t < 0x1.6a09e6p+125f)) res = bq_log1pf_rare(y);   /* k <= 125 */
+    float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
+    float res = __builtin_fmaf(ef, BQ_LN2_HI_F, __builtin_fmaf(ef, BQ_LN2_LO_F, r));
+    /* 0 < t < sqrt(2) 2^125 (y > -1, k <= 125, not NaN) as one unsigned compare of the bits */
+    if (bq_fbits(t) - 1u >= 0x7E3504F3u - 1u) res = bq_log1pf_rare(y);
     return res;
 }
 
