@@ -96,9 +96,9 @@ BQ_FN T bq_tmh2_##SFX(T x, T phi, T r, T dlt, T c12, T l1)                      
  * dop = d/phi, iphip = 1/phi', lrat = log(phi'/phi), dlg0 = lgamma(phi')-lgamma(phi) */ \
 BQ_FN T bq_tphi_##SFX(int xi, T x, T phi, T phip, T mu, T d, T dop, T iphip, T lrat, T dlg0) \
 {                                                                                   \
-    T a = (phi + x) * LOG1P((dop * mu) / (phip + mu));                             \
-    T b = d * LOG1P(mu * iphip);                                                   \
-    T t = (a - b) - x * lrat;                                                      \
+    T l1 = LOG1P((dop * mu) / (phip + mu));                                        \
+    T l2 = LOG1P(mu * iphip);                                                      \
+    T t = FMA(-d, l2, FMA(phi + x, l1, -(x * lrat)));                              \
     if (xi > 0) t = t + (LGDIFF(phi + x, d) - dlg0);                               \
     return t;                                                                      \
 }

@@ -409,7 +409,7 @@ BQ_FN float bq_lgamma_diff32(float z, float d)
     /* both reciprocals from one division (z, zp >= 4: no overflow; ~2 ulp each) */
     float w = 1.0f / (z * zp);
     float iz = zp * w, izp = z * w;
-    float core = d * bq_logf_norm(zp) + ((z - 0.5f) * bq_log1pf(d * iz) - d);
+    float core = __builtin_fmaf(d, bq_logf_norm(zp), __builtin_fmaf(z - 0.5f, bq_log1pf(d * iz), -d));
     float dl = core + (bq_stirling_s32(izp) - bq_stirling_s32(iz));
     if (shifted) dl = dl - bq_logf(pzp / pz);
     return dl;
@@ -422,7 +422,7 @@ BQ_FN double bq_lgamma_diff64(double z, double d)
     int shifted = 0;
     while (z < 10.0 || zp < 10.0) { pz *= z; pzp *= zp; z += 1.0; zp += 1.0; shifted = 1; }
     double iz = 1.0 / z, izp = 1.0 / zp;
-    double core = d * bq_log(zp) + ((z - 0.5) * bq_log1p(d * iz) - d);
+    double core = __builtin_fma(d, bq_log(zp), __builtin_fma(z - 0.5, bq_log1p(d * iz), -d));
     double dl = core + (bq_stirling_s64(izp) - bq_stirling_s64(iz));
     if (shifted) dl = dl - (bq_log(pzp) - bq_log(pz));
     return dl;

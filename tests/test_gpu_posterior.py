@@ -13,9 +13,11 @@ but a few percent of the parameters (standard errors from the effective
 sample size, Geyer's initial positive sequence).  The cell effects delta of a
 batch and the batch effects nu of that batch have one slowly mixing direction
 -- all delta of the batch up by c, all its nu down by c, which moves only the
-batch's first cell (its delta is fixed at 0, cu:766-772) -- so delta and nu
-are compared after removing their per-batch mean in every recorded iteration,
-and the per-batch shift itself is bounded separately.  The dropout curve gamma
+batch's first cell (its delta is fixed at 0, cu:766-772); likewise all alpha
+up and all delta down moves only the first cells -- so alpha, delta and nu are
+compared after removing their mean (over genes for alpha, over the batch for
+delta and nu) in every recorded iteration, and those means are bounded
+separately.  The dropout curve gamma
 (2 parameters per batch) mixes slowly with the imputed counts -- a few
 effective draws per chain, too few for a Monte-Carlo error -- and is held to
 3 posterior standard deviations instead.
@@ -87,6 +89,8 @@ def test_mixed_precision_posterior_agrees_with_fp64():
 
     def centred(p, key):
         tr = p["traces"][key].copy()
+        if key == "alpha":        # minus the mean over genes, per iteration
+            tr -= tr.mean(0)
         if key == "delta":        # minus the batch mean of the cells' delta, per iteration
             for b in range(len(n_b)):
                 tr[batch == b] -= tr[batch == b].mean(0)
@@ -114,5 +118,6 @@ def test_mixed_precision_posterior_agrees_with_fp64():
             assert np.all(d[live] <= 3 * sd[live]), key
         assert np.all(d[~live] == 0), key
     shift = [abs(p32["delta"][batch == b].mean() - p64["delta"][batch == b].mean()) for b in range(len(n_b))]
-    print("per-batch shift of mean delta:", np.round(shift, 4))
+    shift.append(abs(p32["alpha"].mean() - p64["alpha"].mean()))
+    print("shift of mean delta per batch and of mean alpha:", np.round(shift, 4))
     assert max(shift) < 0.05
