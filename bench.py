@@ -45,10 +45,11 @@ def log(*a):
 def kernel_bytes(name, N, G, n_zero):
     """Algorithmic HBM bytes of one launch.  The streaming passes read the int32
     state matrix S once (4 B per cell x gene).  k_xz touches only the Y = 0
-    entries: their uint16 gene index and their S word (6 B read) and writes S
-    back when it changes (at most 4 B)."""
+    entries: their uint16 gene index and their compact state word (6 B read),
+    and writes the state to S and to the compact copy when it changes (at most
+    8 B)."""
     if name == "k_xz":
-        return 10.0 * n_zero
+        return 14.0 * n_zero
     if name in ("k_w", "k_alpha", "k_beta_nu", "k_delta", "k_phi"):
         return 4.0 * N * G
     return None
