@@ -35,6 +35,10 @@ CONFIGS = {
                data_seed=1003, g0=(-1.0, 0.5)),
 }
 HBM_PEAK_GBS = 8000.0
+# Every timed sweep is a sweep of the recorded phase of a run (iterations past
+# burn-in and past n_unchanged): the p/tau0 update and the posterior
+# accumulation run on every sweep, as in the second half of a 4000-iteration run.
+PHASE = dict(n_iter=4000, n_burnin=0, n_unchanged=0)
 METRIC = "Gibbs sweeps/sec and cell×gene updates/s at N=20k,G=3k,K=7; batched vs plain"
 
 
@@ -72,8 +76,7 @@ def pmc_profile(kernel, cfg_name, key):
 def cpu_baseline(Y, cfg, precision, budget_s):
     from oracle import pyoracle as po
     threads = min(16, os.cpu_count() or 1)
-    o = po.Oracle(Y, cfg["n_b"], cfg["K"], seed=13579, precision=precision, n_iter=4000, n_burnin=2000,
-                  n_unchanged=500, threads=threads)
+    o = po.Oracle(Y, cfg["n_b"], cfg["K"], seed=13579, precision=precision, threads=threads, **PHASE)
     t0 = time.perf_counter()
     o.sweeps(1)
     one = time.perf_counter() - t0
@@ -83,7 +86,7 @@ def cpu_baseline(Y, cfg, precision, budget_s):
     dt = time.perf_counter() - t0
     o.close()
     return {"value": n / dt, "unit": "Gibbs sweeps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} sweeps of the same workload after 1 warm-up sweep, CPU oracle "
+            "sample": f"{n} recorded-phase sweeps of the same workload after 1 warm-up sweep, CPU oracle "
                       f"(oracle/oracle.c, OpenMP, {threads} threads, precision {precision})"}
 
 
@@ -96,8 +99,7 @@ def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
     Y = bq.simulate(n_b, G, cfg["K"], cfg["data_seed"], gamma0_range=cfg["g0"])
     Ks = list(range(3, 11))
     chains = 4
-    probs = [dict(Y=Y, n_b=n_b, K=K, seed=13579, chains=chains, n_iter=4000, n_burnin=2000, n_unchanged=500)
-             for K in Ks]
+    probs = [dict(Y=Y, n_b=n_b, K=K, seed=13579, chains=chains, **PHASE) for K in Ks]
     tr_b, _ = bq.autotune_batch(probs, precision=precision)
     sb = bq.Sampler.batch(probs, precision=precision, tile_rows=tr_b)
     sb.sweeps(warmup)
@@ -106,9 +108,10 @@ def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
     total_plain_ms = 0.0
     tr_p = {}
     for K in Ks:
-        tr_p[K], _ = bq.autotune(Y, n_b, K, seed=13579, precision=precision)
+        tr_p[K], _ = bq.autotune(Y, n_b, K, seed=13579, precision=precision, n_burnin=0, n_unchanged=0)
         for c in range(chains):
-            with bq.Sampler(Y, n_b, K, seed=13579, chain0=c, precision=precision, tile_rows=tr_p[K]) as sp:
+            with bq.Sampler(Y, n_b, K, seed=13579, chain0=c, precision=precision, tile_rows=tr_p[K],
+                            **PHASE) as sp:
                 sp.sweeps(warmup)
                 total_plain_ms += sp.time_sweeps(sweeps)
     n_chains = len(Ks) * chains
@@ -146,7 +149,7 @@ def heterogeneous_batch(bq, precision, sweeps=20, warmup=3):
     probs = []
     for d in sets:
         Y = bq.simulate(d["n_b"], d["G"], d["K"], d["seed"])
-        probs.append(dict(Y=Y, n_b=d["n_b"], K=d["K"], seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500))
+        probs.append(dict(Y=Y, n_b=d["n_b"], K=d["K"], seed=13579, **PHASE))
     cells = sum(sum(d["n_b"]) * d["G"] for d in sets)
     tr_b, _ = bq.autotune_batch(probs, precision=precision, sweeps=5)
     sb = bq.Sampler.batch(probs, precision=precision, tile_rows=tr_b)
@@ -155,8 +158,10 @@ def heterogeneous_batch(bq, precision, sweeps=20, warmup=3):
     sb.close()
     ms_p = 0.0
     for p in probs:
-        tr, _ = bq.autotune(p["Y"], p["n_b"], p["K"], seed=13579, precision=precision, sweeps=5)
-        with bq.Sampler(p["Y"], p["n_b"], p["K"], seed=13579, precision=precision, tile_rows=tr) as sp:
+        tr, _ = bq.autotune(p["Y"], p["n_b"], p["K"], seed=13579, precision=precision, sweeps=5,
+                            n_burnin=0, n_unchanged=0)
+        with bq.Sampler(p["Y"], p["n_b"], p["K"], seed=13579, precision=precision, tile_rows=tr,
+                        **PHASE) as sp:
             sp.sweeps(warmup)
             ms_p += sp.time_sweeps(sweeps)
     return {"config": "c5: 16 data sets, N 500-30000 log-uniform, G 1000-5000, B 2-6, K 3-10",
@@ -199,10 +204,9 @@ def main():
     calib_ms = bq.copy_probe(1 << 30)
     log(f"[bench] copy probe 1 GiB: {calib_ms:.3f} ms = {2 * (1 << 30) / calib_ms / 1e6:.0f} GB/s (read+write)")
 
-    tr, _ = bq.autotune(Y, n_b, K, seed=13579, precision=args.precision)
+    tr, _ = bq.autotune(Y, n_b, K, seed=13579, precision=args.precision, n_burnin=0, n_unchanged=0)
     t0 = time.perf_counter()
-    s = bq.Sampler(Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500, precision=args.precision,
-                   tile_rows=tr)
+    s = bq.Sampler(Y, n_b, K, seed=13579, precision=args.precision, tile_rows=tr, **PHASE)
     log(f"[bench] init {time.perf_counter() - t0:.1f}s, tile rows {tr} (autotuned)")
     s.sweeps(args.warmup)
     w0 = time.perf_counter()
@@ -262,7 +266,8 @@ def main():
         "data": "synthetic (BUSseq generative model, seeded)",
         "config": {"workload": f"plain BUSseq Gibbs sweep, one chain, {args.config}: N={N} "
                                f"({'/'.join(map(str, n_b))}), G={G}, K={K}, B={len(n_b)}",
-                   "chains": 1, "precision": args.precision, "tile_rows": tr, "parallelism": "single GPU"},
+                   "chains": 1, "precision": args.precision, "tile_rows": tr, "parallelism": "single GPU",
+                   "phase": "recorded phase: p/tau0 update and posterior accumulation on every sweep"},
         "roofline": roof,
         "sweep_roofline": {"bytes_per_sweep": sweep_bytes,
                            "achieved": round(sweep_bytes * sweeps_per_s / 1e9, 2),

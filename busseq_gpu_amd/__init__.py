@@ -326,9 +326,12 @@ def autotune_batch(problems, precision=32, sweeps=20, device=0):
     return tr.value, sps.value
 
 
-def autotune(Y, n_b, K, seed=13579, chains=1, precision=32, sweeps=20, device=0):
-    """Fastest tile size for one problem (busseq_autotune); returns (tile_rows, sweeps_per_s)."""
-    return autotune_batch([dict(Y=Y, n_b=n_b, K=K, seed=seed, chains=chains)], precision, sweeps, device)
+def autotune(Y, n_b, K, seed=13579, chains=1, precision=32, sweeps=20, device=0, n_burnin=2000,
+             n_unchanged=500):
+    """Fastest tile size for one problem (busseq_autotune); returns (tile_rows, sweeps_per_s).
+    n_burnin and n_unchanged set which phase of a run the timed sweeps belong to."""
+    return autotune_batch([dict(Y=Y, n_b=n_b, K=K, seed=seed, chains=chains, n_burnin=n_burnin,
+                                n_unchanged=n_unchanged)], precision, sweeps, device)
 
 
 def run(Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500, keep_traces=False,

@@ -61,6 +61,7 @@
     Q uint32_t* hist_d;  /* [B*H] dynamic histogram (entries with Y = 0), values >= BQ_HLO */ \
     Q uint32_t* hist_lo;  /* [BQ_STRIPES][B*BQ_HLO] the same for values < BQ_HLO, striped */ \
     Q uint64_t* gstat;  /* [BQ_STRIPES][B*5] n_z1, sum x z, over0, over1_rej, over1_acc, striped */ \
+    Q uint64_t* lstat;  /* [2] sum of L and exact sum of beta^2 over L = 0 (k >= 1), for p and tau0 */ \
     Q double* sp_tab;  /* [B*H] log(1 + e^{-(gamma0 + gamma1 x)}) for the current gamma */ \
     Q int32_t* xz_tail;  /* [nz] per cell, at the cell's zero-entry offset: entries left to the gamma-Poisson draw */ \
     Q int32_t* xz_ntail;  /* [N] their number per cell this sweep */ \

@@ -1,14 +1,15 @@
 // engine.hip -- gfx950 kernels of one BUSseq Gibbs sweep and the host driver
 // that allocates the chains, captures the sweep into a hipGraph and replays it.
 //
-// Sweep (one hipGraph, fifteen launches, chains on blockIdx.z):
+// Sweep (one hipGraph, sixteen launches, chains on blockIdx.z):
 //   k_prep       alpha/gamma proposals, zero the per-sweep counters
 //   k_w          W_i: categorical full conditional, one wave per cell     [row pass]
-//   k_plan       pi_b Dirichlet draw, stable sort of cells by (b, W), tiles
+//   k_plan       stable sort of cells by (b, W), tiles
 //   k_xz         X/Z imputation of the Y = 0 entries, gamma histogram     [row pass]
+//   k_xz_tail    the X/Z entries with a gamma-Poisson proposal
 //   k_alpha      alpha partial sums                                      [tile pass]
 //   k_alpha_dec  alpha accept, L draw, beta/nu/phi proposals
-//   k_gamma_ptau gamma0/gamma1 accept per batch; p, tau0 draw
+//   k_gamma_ptau gamma0/gamma1 accept per batch; pi_b Dirichlet draw; p, tau0 draw
 //   k_beta_nu    beta and nu partial sums (nu under both beta outcomes)  [tile pass]
 //   k_b_dec      beta accept, per (gene, subtype)
 //   k_n_dec      nu accept with the accepted beta branch, per (gene, batch)
@@ -89,6 +90,24 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* sh)
     return r;
 }
 
+// block-wide sums of three values (blockDim.x == 256)
+__device__ __forceinline__ void block_sum3_u64(uint64_t& a, uint64_t& b, uint64_t& d, uint64_t (*sh)[4])
+{
+    a = wave_sum_u64(a);
+    b = wave_sum_u64(b);
+    d = wave_sum_u64(d);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = a;
+        sh[1][threadIdx.x >> 6] = b;
+        sh[2][threadIdx.x >> 6] = d;
+    }
+    __syncthreads();
+    a = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    b = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+    d = (sh[2][0] + sh[2][1]) + (sh[2][2] + sh[2][3]);
+}
+
 // atomics on global-space addresses (the cast back to a generic pointer is
 // undone by the compiler's address-space inference: the instruction is global)
 template <typename V>
@@ -153,6 +172,7 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
         c.sp_tab[j] = bq_neg_logit_sp(c.gamma[b], c.gamma[c.B + b], (double)x);
     }
     if (j < c.B * c.K * BQ_STRIPES) c.cnt[j] = 0;
+    if (j < 2) c.lstat[j] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -238,20 +258,21 @@ void launch_w(int K, dim3 grid, dim3 blk, hipStream_t st, const ChainDev* cd)
 }
 
 // ---------------------------------------------------------------------------
-// k_plan: pi_b ~ Dirichlet(n_b. + xi) (cu:856-890, normalisation race fixed),
-// stable partition of batch b's cells by subtype, tile list.  One block per batch.
+// k_plan: stable partition of batch b's cells by subtype (the order every
+// tile pass reads them in) and the tile list.  One block of PLAN_THREADS
+// threads per batch; thread t owns a contiguous run of the batch's cells.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
+#define PLAN_THREADS 1024
+__global__ __launch_bounds__(PLAN_THREADS) void k_plan(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int b = blockIdx.x;
     if (b >= c.B) return;
     const int K = c.K, B = c.B, TR = c.TR, tid = threadIdx.x;
-    const int t = *c.iter;
     __shared__ int s_toff[BQ_BMAX * BQ_KMAX + 1];
     __shared__ int s_cnt[BQ_BMAX * BQ_KMAX];
-    __shared__ int s_pos[256 * BQ_KMAX];
-    for (int j = tid; j < B * K; j += 256) {
+    __shared__ int s_wsum[PLAN_THREADS / 64][BQ_KMAX];
+    for (int j = tid; j < B * K; j += PLAN_THREADS) {
         s_cnt[j] = sum_stripes<int>((const int32_t*)c.cnt + j, (size_t)B * K);
     }
     __syncthreads();
@@ -262,7 +283,7 @@ __global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
     }
     __syncthreads();
     if (b == 0) {
-        for (int j = tid; j <= B * K; j += 256) c.tile_off[j] = s_toff[j];
+        for (int j = tid; j <= B * K; j += PLAN_THREADS) c.tile_off[j] = s_toff[j];
         if (tid == 0) *c.ntiles = s_toff[B * K];
     }
     // tiles of this batch's groups
@@ -270,7 +291,7 @@ __global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
     int gstart = b0;
     for (int k = 0; k < K; ++k) {
         int n = s_cnt[b * K + k], t0 = s_toff[b * K + k];
-        for (int j = tid; j * TR < n; j += 256) {
+        for (int j = tid; j * TR < n; j += PLAN_THREADS) {
             int q = t0 + j;
             c.tiles[4 * q] = b;
             c.tiles[4 * q + 1] = k;
@@ -279,44 +300,64 @@ __global__ __launch_bounds__(256) void k_plan(const ChainDev* __restrict__ cds)
         }
         gstart += n;
     }
-    // pi_b
-    if (tid == 0) {
-        bq_rng r;
-        bq_rng_init(&r, c.seed, c.key, (uint32_t)b, 0u, (uint32_t)t, BQ_PH_PI);
-        double gk[BQ_KMAX], tot = 0.0;
-        for (int k = 0; k < K; ++k) { gk[k] = bq_rgamma(&r, s_cnt[b * K + k] + BQ_XI_PI, 1.0); tot += gk[k]; }
-        for (int k = 0; k < K; ++k) c.pi[b * K + k] = gk[k] / tot;
-    }
     // stable partition: thread tid owns cells [lo, hi) of the batch
-    const int chunk = (nb + 255) / 256;
+    const int chunk = (nb + PLAN_THREADS - 1) / PLAN_THREADS;
     const int lo = b0 + tid * chunk;
     const int hi = lo + chunk < b0 + nb ? lo + chunk : b0 + nb;
     int my[BQ_KMAX];
 #pragma unroll
     for (int k = 0; k < BQ_KMAX; ++k) my[k] = 0;
-    for (int i = lo; i < hi; ++i) {
-        int w = c.W[i];
+    // subtypes of the owned cells, eight loads in flight
+    for (int i0 = lo; i0 < hi; i0 += 8) {
+        int wb[8];
 #pragma unroll
-        for (int k = 0; k < BQ_KMAX; ++k) my[k] += (w == k);
+        for (int u = 0; u < 8; ++u) wb[u] = i0 + u < hi ? c.W[i0 + u] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int k = 0; k < BQ_KMAX; ++k) my[k] += (wb[u] == k);
     }
+    // exclusive prefix of the counts over the threads, per subtype: a scan
+    // within each wave, then the totals of the waves before
+    const int lane = tid & 63, wv = tid >> 6;
+    int incl[BQ_KMAX];
 #pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k) s_pos[k * 256 + tid] = my[k];
-    __syncthreads();
-    if (tid < K) {
-        int acc = b0;
-        for (int k = 0; k < tid; ++k) acc += s_cnt[b * K + k];
-        for (int q = 0; q < 256; ++q) { int v = s_pos[tid * 256 + q]; s_pos[tid * 256 + q] = acc; acc += v; }
+    for (int k = 0; k < BQ_KMAX; ++k) {
+        int v = my[k];
+        if (k < K) {
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int o = __shfl_up(v, d);
+                if (lane >= d) v += o;
+            }
+            if (lane == 63) s_wsum[wv][k] = v;
+        }
+        incl[k] = v;
     }
     __syncthreads();
+    int base = b0;
 #pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k) my[k] = s_pos[k * 256 + tid];
-    for (int i = lo; i < hi; ++i) {
-        int w = c.W[i];
-        int pos = 0;
+    for (int k = 0; k < BQ_KMAX; ++k) {
+        if (k < K) {
+            int off = base;
+            for (int w = 0; w < wv; ++w) off += s_wsum[w][k];
+            my[k] = off + incl[k] - my[k];
+            base += s_cnt[b * K + k];
+        }
+    }
+    for (int i0 = lo; i0 < hi; i0 += 8) {
+        int wb[8];
 #pragma unroll
-        for (int k = 0; k < BQ_KMAX; ++k)
-            if (w == k) { pos = my[k]; my[k] += 1; }
-        c.perm[pos] = i;
+        for (int u = 0; u < 8; ++u) wb[u] = i0 + u < hi ? c.W[i0 + u] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (wb[u] < 0) break;
+            int pos = 0;
+#pragma unroll
+            for (int k = 0; k < BQ_KMAX; ++k)
+                if (wb[u] == k) { pos = my[k]; my[k] += 1; }
+            c.perm[pos] = i0 + u;
+        }
     }
 }
 
@@ -643,62 +684,44 @@ __global__ __launch_bounds__(256) void k_alpha(const ChainDev* __restrict__ cds)
     c.part[((size_t)tile * 3) * c.Gp + g] = bq_fix(acc);
 }
 
-// exact sum of the fixed-point tile partials p[tt * stride], tt in [t0, t1),
-// with eight loads in flight (wrapping integer adds: the order does not matter)
-__device__ __forceinline__ uint64_t sum_tiles(const uint64_t* p, int t0, int t1, size_t stride)
+// exact sum of the fixed-point tile partials p[m * stride], m in [0, n), with
+// eight loads in flight (wrapping integer adds: the order does not matter)
+__device__ __forceinline__ uint64_t sum_tiles_n(const uint64_t* p, int n, size_t stride)
 {
     uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int tt = t0;
-    for (; tt + 8 <= t1; tt += 8) {
+    int m = 0;
+    for (; m + 8 <= n; m += 8) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] += p[(size_t)(tt + u) * stride];
+        for (int u = 0; u < 8; ++u) a[u] += p[(size_t)(m + u) * stride];
     }
-    for (; tt < t1; ++tt) a[0] += p[(size_t)tt * stride];
+    for (; m < n; ++m) a[0] += p[(size_t)m * stride];
     return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
-// ---------------------------------------------------------------------------
-// k_alpha_dec: flat work items
-//   [0, G)                 alpha accept (cu:625-632)
-//   + G(K-1)  (g, k>=1)    L Gibbs draw (cu:634-648), beta proposal (cu:670-674)
-//   + G(B-1)  (g, b>=1)    nu proposal (cu:709-713)
-//   + G B     (g, b)       phi proposal (cu:780-787) and the phi-term constants
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ cds)
+// The decision kernels run one block per 64 consecutive columns (genes) of an
+// item class; the block's four waves split the tiles of each column and add
+// their partials in LDS.  sum_tiles_w is wave w's share of the tiles [t0, t1):
+// t0 + w, t0 + w + 4, ...
+__device__ __forceinline__ uint64_t sum_tiles_w(const uint64_t* p, int t0, int t1, size_t stride, int w)
 {
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    const int G = c.G, K = c.K, B = c.B;
-    int j = blockIdx.x * 256 + threadIdx.x;
-    const int t = *c.iter;
-    if (j < G) {
-        const int g = j, nt = *c.ntiles;
-        double sum = bq_unfix(sum_tiles((const uint64_t*)(c.part + g), 0, nt, (size_t)3 * c.Gp));
-        double al = c.alpha[g], ap = c.alpha_p[g];
-        double lr = sum + bq_lr_normal(al, ap, c.mu_alpha[g], BQ_SIGMA2_ALPHA);
-        if (bq_log(ph_uniform(c, (uint32_t)g, 0u, t, BQ_PH_ALPHA_A)) <= lr) {
-            c.alpha[g] = ap;
-            ((T*)c.ea)[g] = (T)bq_exp(ap);
-        }
-        return;
-    }
-    j -= G;
-    if (j < G * (K - 1)) {
-        const int k = 1 + j / G, g = j % G;
-        const size_t q = (size_t)k * G + g;
-        const double be = c.beta[q];
-        c.L[q] = bq_bern_logodds(ph_uniform(c, (uint32_t)g, (uint32_t)k, t, BQ_PH_L),
-                                 bq_logodds_L(c.ptau[0], c.ptau[1], be));
-        bq_rng r;
-        bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)k, (uint32_t)t, BQ_PH_BETA_P);
-        double bp = be + BQ_RW_SD * bq_normal(&r);
-        c.beta_p[q] = bp;
-        double dl = bp - be;
-        ((T*)c.db)[q] = (T)dl;
-        ((T*)c.cb)[q] = (T)bq_expm1(dl);
-        return;
-    }
-    j -= G * (K - 1);
+    const int n = t1 - t0 > w ? (t1 - t0 - w + 3) / 4 : 0;
+    return n ? sum_tiles_n(p + (size_t)(t0 + w) * stride, n, 4 * stride) : 0;
+}
+
+// the four waves' partials of each column added; valid in wave 0
+__device__ __forceinline__ uint64_t combine_waves(uint64_t a, uint64_t (*sp)[64], int wv, int lane)
+{
+    sp[wv][lane] = a;
+    __syncthreads();
+    return (sp[0][lane] + sp[1][lane]) + (sp[2][lane] + sp[3][lane]);
+}
+
+// nu and phi proposals of k_alpha_dec: item j of G(B-1) (g, b>=1) nu items
+// followed by G B (g, b) phi items
+template <typename T>
+__device__ __forceinline__ void dec_proposals(const ChainG& c, int j, int t)
+{
+    const int G = c.G, B = c.B;
     if (j < G * (B - 1)) {
         const int b = 1 + j / G, g = j % G;
         const size_t q = (size_t)b * G + g;
@@ -734,38 +757,114 @@ __global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// k_alpha_dec: the first ceil(G/64) blocks decide alpha (cu:625-632), 64 genes
+// each; the blocks after them take flat work items
+//   G(K-1)  (g, k>=1)    L Gibbs draw (cu:634-648), beta proposal (cu:670-674)
+//   + G(B-1)  (g, b>=1)  nu proposal (cu:709-713)
+//   + G B     (g, b)     phi proposal (cu:780-787) and the phi-term constants
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ cds)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    __shared__ uint64_t s_p[4][64];
+    __shared__ uint64_t s_r3[3][4];
+    const int G = c.G, K = c.K;
+    const int nA = (G + 63) / 64;
+    const int t = *c.iter;
+    if ((int)blockIdx.x < nA) {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int g = blockIdx.x * 64 + lane, nt = *c.ntiles;
+        const uint64_t a = g < G ? sum_tiles_w((const uint64_t*)(c.part + g), 0, nt, (size_t)3 * c.Gp, wv) : 0;
+        const uint64_t tot = combine_waves(a, s_p, wv, lane);
+        if (wv != 0 || g >= G) return;
+        double sum = bq_unfix(tot);
+        double al = c.alpha[g], ap = c.alpha_p[g];
+        double lr = sum + bq_lr_normal(al, ap, c.mu_alpha[g], BQ_SIGMA2_ALPHA);
+        if (bq_log(ph_uniform(c, (uint32_t)g, 0u, t, BQ_PH_ALPHA_A)) <= lr) {
+            c.alpha[g] = ap;
+            ((T*)c.ea)[g] = (T)bq_exp(ap);
+        }
+        return;
+    }
+    int j = (blockIdx.x - nA) * 256 + threadIdx.x;
+    // blocks holding L items add sum L and sum beta^2 over L = 0 for the p and
+    // tau0 draws (cu:654-662), with the new L and the beta before its update
+    const bool lblk = (int)(blockIdx.x - nA) * 256 < G * (K - 1);
+    uint64_t lsum = 0, bsq = 0;
+    if (j < G * (K - 1)) {
+        const int k = 1 + j / G, g = j % G;
+        const size_t q = (size_t)k * G + g;
+        const double be = c.beta[q];
+        const int lv = bq_bern_logodds(ph_uniform(c, (uint32_t)g, (uint32_t)k, t, BQ_PH_L),
+                                       bq_logodds_L(c.ptau[0], c.ptau[1], be));
+        c.L[q] = lv;
+        if (lv) lsum = 1;
+        else bsq = bq_fix(be * be);
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)g, (uint32_t)k, (uint32_t)t, BQ_PH_BETA_P);
+        double bp = be + BQ_RW_SD * bq_normal(&r);
+        c.beta_p[q] = bp;
+        double dl = bp - be;
+        ((T*)c.db)[q] = (T)dl;
+        ((T*)c.cb)[q] = (T)bq_expm1(dl);
+    } else {
+        j -= G * (K - 1);
+        dec_proposals<T>(c, j, t);
+    }
+    if (lblk) {
+        uint64_t unused = 0;
+        block_sum3_u64(lsum, bsq, unused, s_r3);
+        if (threadIdx.x == 0) {
+            atomicAdd((unsigned long long*)c.lstat, (unsigned long long)lsum);
+            atomicAdd((unsigned long long*)c.lstat + 1, (unsigned long long)bsq);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_gamma_ptau: gamma0 then gamma1 accept per batch from the X histogram
-// (cu:524-600); the last block draws p and tau0 (cu:650-668)
+// (cu:524-600); pi_b ~ Dirichlet(n_b. + xi) (cu:856-890, normalisation race
+// fixed); p and tau0 (cu:650-668).  Grid: nbx gamma blocks (one per batch),
+// nbx pi blocks, one p/tau0 block.  pi is read only by the next sweep's subtype
+// draw and by the recorder, so it is drawn here, beside the gamma blocks.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    __shared__ uint64_t s_red[4];
-    __shared__ double s_g0;
-    __shared__ int s_acc0;
+    __shared__ uint64_t s_red[3][4];
+    __shared__ int s_cnt[BQ_KMAX];
     __shared__ int64_t s_n[H];        // histogram of X over the batch's entries (static + dynamic)
     __shared__ uint64_t s_gs[5];      // the batch's gamma statistics, stripes added
     const int tid = threadIdx.x, B = c.B, t = *c.iter;
+    const int nbx = (int)(gridDim.x - 1) / 2;
     if (blockIdx.x == gridDim.x - 1) {
-        if (t < c.n_unchanged) return;
+        if (t < c.n_unchanged || tid != 0) return;
         const int G = c.G, K = c.K;
-        uint64_t cl = 0, sb = 0;
-        for (int q = G + tid; q < K * G; q += 256) {
-            if (c.L[q]) cl += 1;
-            else { double be = c.beta[q]; sb += bq_fix(be * be); }
+        const double n = (double)G * (K - 1);
+        const double cnt = (double)(int64_t)c.lstat[0];
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, 0u, 0u, (uint32_t)t, BQ_PH_PTAU);
+        c.ptau[0] = bq_rbeta(&r, cnt + BQ_A_P, n - cnt + BQ_B_P);
+        double a = BQ_A_TAU0 + (n - cnt) / 2.0;
+        double bb = BQ_B_TAU0 + bq_unfix(c.lstat[1]) / 2.0;
+        c.ptau[1] = 1.0 / bq_rgamma(&r, a, 1.0 / bb);
+        return;
+    }
+    if ((int)blockIdx.x >= nbx) {
+        const int b = blockIdx.x - nbx, K = c.K;
+        if (b >= B || tid >= 64) return;
+        if (tid < K) s_cnt[tid] = sum_stripes<int>((const int32_t*)c.cnt + b * K + tid, (size_t)B * K);
+        wave_lds_sync();
+        if (tid != 0) return;
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)b, 0u, (uint32_t)t, BQ_PH_PI);
+        double gk[BQ_KMAX], tot = 0.0;
+        for (int k = 0; k < K; ++k) {
+            gk[k] = bq_rgamma(&r, s_cnt[k] + BQ_XI_PI, 1.0);
+            tot += gk[k];
         }
-        cl = block_sum_u64(cl, s_red);
-        sb = block_sum_u64(sb, s_red);
-        if (tid == 0) {
-            double n = (double)G * (K - 1);
-            double cnt = (double)(int64_t)cl;
-            bq_rng r;
-            bq_rng_init(&r, c.seed, c.key, 0u, 0u, (uint32_t)t, BQ_PH_PTAU);
-            c.ptau[0] = bq_rbeta(&r, cnt + BQ_A_P, n - cnt + BQ_B_P);
-            double a = BQ_A_TAU0 + (n - cnt) / 2.0;
-            double bb = BQ_B_TAU0 + bq_unfix(sb) / 2.0;
-            c.ptau[1] = 1.0 / bq_rgamma(&r, a, 1.0 / bb);
-        }
+        for (int k = 0; k < K; ++k) c.pi[b * K + k] = gk[k] / tot;
         return;
     }
     const int b = blockIdx.x;
@@ -784,46 +883,42 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
         s_gs[tid] = sum_stripes<uint64_t>((const uint64_t*)c.gstat + (size_t)b * 5 + tid, (size_t)B * 5);
     }
     __syncthreads();
+    // the gamma0 sum and the gamma1 sums for both outcomes of the gamma0 step
     const double g0 = c.gamma[b], g1 = c.gamma[B + b];
     const double g0p = c.gamma_p[b], g1p = c.gamma_p[B + b];
-    uint64_t a0 = 0;
+    uint64_t a0 = 0, a1r = 0, a1a = 0;
     for (int x = tid; x < H; x += 256) {
-        int64_t n = s_n[x];
-        if (n) a0 += bq_fix((double)n * (bq_sp_lin(g0, g1, (double)x) - bq_sp_lin(g0p, g1, (double)x)));
+        const int64_t n = s_n[x];
+        if (n) {
+            const double xd = (double)x, nd = (double)n;
+            const double s00 = bq_sp_lin(g0, g1, xd), s10 = bq_sp_lin(g0p, g1, xd);
+            const double s01 = bq_sp_lin(g0, g1p, xd), s11 = bq_sp_lin(g0p, g1p, xd);
+            a0 += bq_fix(nd * (s00 - s10));
+            a1r += bq_fix(nd * (s00 - s01));
+            a1a += bq_fix(nd * (s10 - s11));
+        }
     }
     const int bb0 = c.big_off[b], bb1 = c.big_off[b + 1];
     for (int q = bb0 + tid; q < bb1; q += 256) {
-        double xd = (double)c.big[q];
-        a0 += bq_fix(bq_sp_lin(g0, g1, xd) - bq_sp_lin(g0p, g1, xd));
+        const double xd = (double)c.big[q];
+        const double s00 = bq_sp_lin(g0, g1, xd), s10 = bq_sp_lin(g0p, g1, xd);
+        const double s01 = bq_sp_lin(g0, g1p, xd), s11 = bq_sp_lin(g0p, g1p, xd);
+        a0 += bq_fix(s00 - s10);
+        a1r += bq_fix(s00 - s01);
+        a1a += bq_fix(s10 - s11);
     }
-    a0 = block_sum_u64(a0, s_red);
+    block_sum3_u64(a0, a1r, a1a, s_red);
     if (tid == 0) {
         const uint64_t* gs = s_gs;
         a0 += bq_fix((g0p - g0) * (double)(int64_t)gs[0]);
         a0 += gs[2];
         double s0 = bq_unfix(a0);
         double lr0 = s0 + bq_lr_normal(g0, g0p, 0.0, BQ_SIGMA2_GAMMA0);
-        int acc0 = bq_log(ph_uniform(c, (uint32_t)b, 0u, t, BQ_PH_GAMMA0_A)) <= lr0;
+        const int acc0 = bq_log(ph_uniform(c, (uint32_t)b, 0u, t, BQ_PH_GAMMA0_A)) <= lr0;
         if (acc0) c.gamma[b] = g0p;
-        s_g0 = acc0 ? g0p : g0;
-        s_acc0 = acc0;
-    }
-    __syncthreads();
-    const double g0n = s_g0;
-    uint64_t a1 = 0;
-    for (int x = tid; x < H; x += 256) {
-        int64_t n = s_n[x];
-        if (n) a1 += bq_fix((double)n * (bq_sp_lin(g0n, g1, (double)x) - bq_sp_lin(g0n, g1p, (double)x)));
-    }
-    for (int q = bb0 + tid; q < bb1; q += 256) {
-        double xd = (double)c.big[q];
-        a1 += bq_fix(bq_sp_lin(g0n, g1, xd) - bq_sp_lin(g0n, g1p, xd));
-    }
-    a1 = block_sum_u64(a1, s_red);
-    if (tid == 0) {
-        const uint64_t* gs = s_gs;
+        uint64_t a1 = acc0 ? a1a : a1r;
         a1 += bq_fix((g1p - g1) * (double)(int64_t)gs[1]);
-        a1 += s_acc0 ? gs[4] : gs[3];
+        a1 += acc0 ? gs[4] : gs[3];
         double s1 = bq_unfix(a1);
         double u1 = ph_uniform(c, (uint32_t)b, 0u, t, BQ_PH_GAMMA1_A);
         if (g1p < 0.0) {
@@ -877,21 +972,29 @@ __global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cd
 }
 
 // ---------------------------------------------------------------------------
-// k_b_dec: beta accept (cu:696-707), one item per (g, k >= 1)
+// k_b_dec: beta accept (cu:696-707), one item per (g, k >= 1); block
+// (k - 1) * ceil(G/64) + j takes genes [64 j, 64 j + 64) of subtype k, wave w
+// the groups (b, k) with b = w mod 4
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    __shared__ uint64_t s_p[4][64];
     const int G = c.G, K = c.K, B = c.B;
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= G * (K - 1)) return;
-    const int k = 1 + j / G, g = j % G, t = *c.iter;
+    const int nA = (G + 63) / 64;
+    if ((int)blockIdx.x >= nA * (K - 1)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = 1 + blockIdx.x / nA, g = (blockIdx.x % nA) * 64 + lane, t = *c.iter;
     const size_t Gp = c.Gp;
     auto toff = c.tile_off;
     uint64_t a = 0;
-    for (int b = 0; b < B; ++b)
-        a += sum_tiles((const uint64_t*)(c.part + g), toff[b * K + k], toff[b * K + k + 1], 3 * Gp);
+    if (g < G)
+        for (int b = wv; b < B; b += 4)
+            a += sum_tiles_n((const uint64_t*)(c.part + (size_t)toff[b * K + k] * 3 * Gp + g),
+                             toff[b * K + k + 1] - toff[b * K + k], 3 * Gp);
+    a = combine_waves(a, s_p, wv, lane);
+    if (wv != 0 || g >= G) return;
     const size_t q = (size_t)k * G + g;
     double sum = bq_unfix(a);
     double be = c.beta[q], bp = c.beta_p[q];
@@ -908,23 +1011,31 @@ __global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
 
 // ---------------------------------------------------------------------------
 // k_n_dec: nu accept (cu:730-737), one item per (g, b >= 1), using the nu sums
-// of the accepted beta branch of each subtype
+// of the accepted beta branch of each subtype; block (b - 1) * ceil(G/64) + j
+// takes genes [64 j, 64 j + 64) of batch b, wave w the groups (b, k) with
+// k = w mod 4
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_n_dec(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    __shared__ uint64_t s_p[4][64];
     const int G = c.G, K = c.K, B = c.B;
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= G * (B - 1)) return;
-    const int b = 1 + j / G, g = j % G, t = *c.iter;
+    const int nA = (G + 63) / 64;
+    if ((int)blockIdx.x >= nA * (B - 1)) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int b = 1 + blockIdx.x / nA, g = (blockIdx.x % nA) * 64 + lane, t = *c.iter;
     const size_t Gp = c.Gp;
     auto toff = c.tile_off;
     uint64_t a = 0;
-    for (int k = 0; k < K; ++k) {
-        const int slot = (k >= 1 && c.beta_acc[(size_t)k * G + g]) ? 2 : 1;
-        a += sum_tiles((const uint64_t*)(c.part + slot * Gp + g), toff[b * K + k], toff[b * K + k + 1], 3 * Gp);
-    }
+    if (g < G)
+        for (int k = wv; k < K; k += 4) {
+            const int slot = (k >= 1 && c.beta_acc[(size_t)k * G + g]) ? 2 : 1;
+            a += sum_tiles_n((const uint64_t*)(c.part + ((size_t)toff[b * K + k] * 3 + slot) * Gp + g),
+                             toff[b * K + k + 1] - toff[b * K + k], 3 * Gp);
+        }
+    a = combine_waves(a, s_p, wv, lane);
+    if (wv != 0 || g >= G) return;
     const size_t q = (size_t)b * G + g;
     double sum = bq_unfix(a);
     double nv = c.nu[q], np = c.nu_p[q];
@@ -1029,18 +1140,23 @@ __global__ __launch_bounds__(256) void k_phi(const ChainDev* __restrict__ cds)
 }
 
 // ---------------------------------------------------------------------------
-// k_phi_dec: phi accept (cu:806-818), one item per (g, b)
+// k_phi_dec: phi accept (cu:806-818), one item per (g, b); block
+// b * ceil(G/64) + j takes genes [64 j, 64 j + 64) of batch b
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_phi_dec(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    __shared__ uint64_t s_p[4][64];
     const int G = c.G, K = c.K, B = c.B;
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= G * B) return;
-    const int b = j / G, g = j % G, t = *c.iter;
+    const int nA = (G + 63) / 64;
+    if ((int)blockIdx.x >= nA * B) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int b = blockIdx.x / nA, g = (blockIdx.x % nA) * 64 + lane, t = *c.iter;
     const size_t Gp = c.Gp;
-    const uint64_t a = sum_tiles((const uint64_t*)(c.part + g), c.tile_off[b * K], c.tile_off[(b + 1) * K], 3 * Gp);
+    uint64_t a = g < G ? sum_tiles_w((const uint64_t*)(c.part + g), c.tile_off[b * K], c.tile_off[(b + 1) * K], 3 * Gp, wv) : 0;
+    a = combine_waves(a, s_p, wv, lane);
+    if (wv != 0 || g >= G) return;
     const size_t q = (size_t)b * G + g;
     double sum = bq_unfix(a);
     double f = c.phi[q], fp = c.phi_p[q];
@@ -1171,7 +1287,8 @@ struct ChainHost {
 struct busseq_sampler {
     int C = 0;                     // chains
     int prec = 32, TR = 64, dev = 0;
-    int maxN = 0, maxG = 0, maxK = 0, maxB = 0, maxT = 0, maxPrep = 0, maxItems = 0, maxRec = 0;
+    int maxN = 0, maxG = 0, maxK = 0, maxB = 0, maxT = 0, maxPrep = 0, maxRec = 0;
+    int maxAdec = 0, maxDec = 0;   // blocks of k_alpha_dec and of the other decision kernels
     std::vector<ChainHost> ch;
     std::vector<ChainDev> hd;
     ChainDev* d_cd = nullptr;
@@ -1216,8 +1333,9 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     const dim3 g_rows((s->maxN + 3) / 4, 1, C);
     const dim3 g_plan(s->maxB, 1, C);
     const dim3 g_tile(nblk(s->maxG), s->maxT, C);
-    const dim3 g_items(nblk(s->maxItems), 1, C);
-    const dim3 g_gamma(s->maxB + 1, 1, C);
+    const dim3 g_adec(s->maxAdec, 1, C);
+    const dim3 g_dec(s->maxDec, 1, C);
+    const dim3 g_gamma(2 * s->maxB + 1, 1, C);
     const dim3 g_rec(nblk(s->maxRec), 1, C);
     int q = 0;
 #define EV() do { if (ev) HIPCHK(hipEventRecord(ev[q], st)); ++q; } while (0)
@@ -1228,18 +1346,18 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
         for (int c = kr.first; c < kr.first + kr.second; ++c) maxn = s->ch[c].N > maxn ? s->ch[c].N : maxn;
         launch_w<T>(s->ch[kr.first].K, dim3((maxn + 3) / 4, 1, kr.second), blk, st, cd + kr.first);
     }
-    EV(); hipLaunchKernelGGL(k_plan, g_plan, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_plan, g_plan, dim3(PLAN_THREADS), 0, st, cd);
     EV(); hipLaunchKernelGGL(k_xz<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_xz_tail<T>, dim3((s->maxN + 4 * TAIL_CPW - 1) / (4 * TAIL_CPW), 1, C), blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_items, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_adec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_beta_nu<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_b_dec<T>, g_items, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_n_dec<T>, g_items, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_b_dec<T>, g_dec, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_n_dec<T>, g_dec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_phi_dec<T>, g_items, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_phi_dec<T>, g_dec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_record, g_rec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_tick, dim3(1, 1, C), dim3(64), 0, st, cd);
     EV();
@@ -1454,6 +1572,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.hist_d = s->dalloc<uint32_t>((size_t)B * H);
             c.hist_lo = s->dalloc<uint32_t>((size_t)B * BQ_HLO * BQ_STRIPES);
             c.gstat = s->dalloc<uint64_t>((size_t)B * 5 * BQ_STRIPES);
+            c.lstat = s->dalloc<uint64_t>(2);
             c.xz_tail = s->dalloc<int32_t>(nz > 0 ? (size_t)nz : 1);
             c.zs = s->dalloc<int32_t>(nz > 0 ? (size_t)nz : 1);   // every Y = 0 entry starts at X = 0, Z = 0
             c.xz_ntail = s->dalloc<int32_t>(N);
@@ -1494,7 +1613,9 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             auto mx = [](int& a, long v) { if (v > a) a = (int)v; };
             mx(s->maxN, N); mx(s->maxG, G); mx(s->maxK, K); mx(s->maxB, B); mx(s->maxT, T_max);
             mx(s->maxPrep, G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * BQ_HLO * BQ_STRIPES); mx(s->maxPrep, (long)B * K * BQ_STRIPES);
-            mx(s->maxItems, (long)G * (1 + (K - 1) + (B - 1) + B));
+            const long nA = (G + 63) / 64;
+            mx(s->maxAdec, nA + nblk((long)G * ((K - 1) + (B - 1) + B)));
+            mx(s->maxDec, nA * (K - 1)); mx(s->maxDec, nA * (B - 1)); mx(s->maxDec, nA * B);
             mx(s->maxRec, N); mx(s->maxRec, (long)K * G); mx(s->maxRec, (long)B * G); mx(s->maxRec, (long)B * K);
             mx(s->maxRec, 2L * B);
         }

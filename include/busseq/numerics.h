@@ -505,19 +505,20 @@ BQ_FN double bq_rgamma(bq_rng* r, double a, double scale)
     return g * scale;
 }
 
-/* 1/k for k = 0..255 (entry 0 unused), correctly rounded doubles */
+/* 1/k for k = 0..256 (entry 0 unused), correctly rounded doubles */
 #ifdef __HIPCC__
 #define BQ_TABLE static __constant__ const
 #else
 #define BQ_TABLE static const
 #endif
-#define BQ_INV_N 256
+#define BQ_INV_N 257
 BQ_TABLE double bq_inv_tab[BQ_INV_N] = {
 #define BQ_I4(a) 1.0 / (a), 1.0 / (a + 1), 1.0 / (a + 2), 1.0 / (a + 3)
 #define BQ_I16(a) BQ_I4(a), BQ_I4(a + 4), BQ_I4(a + 8), BQ_I4(a + 12)
     0.0, 1.0, 1.0 / 2, 1.0 / 3, BQ_I4(4), BQ_I4(8), BQ_I4(12),
     BQ_I16(16), BQ_I16(32), BQ_I16(48), BQ_I16(64), BQ_I16(80), BQ_I16(96), BQ_I16(112),
-    BQ_I16(128), BQ_I16(144), BQ_I16(160), BQ_I16(176), BQ_I16(192), BQ_I16(208), BQ_I16(224), BQ_I16(240)
+    BQ_I16(128), BQ_I16(144), BQ_I16(160), BQ_I16(176), BQ_I16(192), BQ_I16(208), BQ_I16(224), BQ_I16(240),
+    1.0 / 256
 #undef BQ_I16
 #undef BQ_I4
 };
@@ -581,10 +582,13 @@ BQ_FN int bq_rnbinom_inv_t(double u, double mu, double phi, const double* inv)
     double p = bq_exp(phi * bq_log1p(-q));
     double s = p;
     int k = 0;
+    double ik = inv[1];   /* 1/k of the next step, read one step ahead */
     while (u > s && k < 255) {
         k += 1;
-        p *= q * ((phi + (double)(k - 1)) * inv[BQ_WAVE_UNIFORM(k)]);
+        const double ikn = inv[BQ_WAVE_UNIFORM(k + 1)];
+        p *= q * ((phi + (double)(k - 1)) * ik);
         s += p;
+        ik = ikn;
     }
     return k;
 }
