@@ -50,6 +50,7 @@
     Q double *alpha_p, *beta_p, *nu_p, *phi_p, *gamma_p; \
     Q void *da, *ca, *db, *cb, *dn, *cn; \
     Q void* pc;  /* [6][B*G]: phi', d, d/phi, 1/phi', log(phi'/phi), lgamma diff */ \
+    Q void* tm;  /* [K*B*G] (e^alpha e^beta_k) e^nu_b in precision T, the X/Z step's mean without e^delta */ \
     /* plan of the sweep */ \
     Q int32_t* cnt;  /* [BQ_STRIPES][B*K] cells per (batch, subtype), striped */ \
     Q int32_t* perm;  /* [N] cells sorted stably by (batch, subtype, index) */ \
@@ -62,7 +63,7 @@
     Q uint32_t* hist_lo;  /* [BQ_STRIPES][B*BQ_HLO] the same for values < BQ_HLO, striped */ \
     Q uint64_t* gstat;  /* [BQ_STRIPES][B*5] n_z1, sum x z, over0, over1_rej, over1_acc, striped */ \
     Q uint64_t* lstat;  /* [2] sum of L and exact sum of beta^2 over L = 0 (k >= 1), for p and tau0 */ \
-    Q double* sp_tab;  /* [B*H] log(1 + e^{-(gamma0 + gamma1 x)}) for the current gamma */ \
+    Q double* odds_tab;  /* [B*H] bq_xz_odds(gamma0, gamma1, x) = 1 + e^{-(gamma0 + gamma1 x)}, current gamma */ \
     Q int32_t* xz_tail;  /* [nz] per cell, at the cell's zero-entry offset: entries left to the gamma-Poisson draw */ \
     Q int32_t* xz_ntail;  /* [N] their number per cell this sweep */ \
     Q int32_t* beta_acc;  /* [K*G] 1 where this sweep accepted beta_kg */ \

@@ -136,7 +136,8 @@ __device__ __forceinline__ double ph_uniform(const ChainG& c, uint32_t a, uint32
 
 // ---------------------------------------------------------------------------
 // k_prep: alpha and gamma proposals (cu:514-522, 602-606), counters to zero,
-// table of the X/Z acceptance terms log(1 + e^{-(gamma0 + gamma1 x)}), x < H
+// table of the X/Z acceptance factors 1 + e^{-(gamma0 + gamma1 x)}, x < H,
+// and the X/Z step's per-(subtype, batch, gene) mean factors
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
@@ -169,10 +170,15 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
     if (j < c.B * H) {
         c.hist_d[j] = 0u;
         const int b = j / H, x = j - b * H;
-        c.sp_tab[j] = bq_neg_logit_sp(c.gamma[b], c.gamma[c.B + b], (double)x);
+        c.odds_tab[j] = bq_xz_odds(c.gamma[b], c.gamma[c.B + b], (double)x);
     }
     if (j < c.B * c.K * BQ_STRIPES) c.cnt[j] = 0;
     if (j < 2) c.lstat[j] = 0;
+    if (j < c.K * c.B * c.G) {
+        // ((e^alpha e^beta) e^nu): the first two products of the mean, in its order
+        const int G = c.G, g = j % G, kb = j / G, b = kb % c.B, k = kb / c.B;
+        ((T*)c.tm)[j] = (((const T*)c.ea)[g] * ((const T*)c.eb)[(size_t)k * G + g]) * ((const T*)c.en)[(size_t)b * G + g];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -404,24 +410,27 @@ __device__ __forceinline__ void xz_big_terms(double g0, double g1, double g0p, d
 }
 
 static_assert(BQ_HLO == 64, "k_xz keeps one low histogram bin per lane");
-#define XZ_SPT 256   // softplus table entries staged in LDS (every inversion draw is below it)
+#define XZ_SPT 256   // acceptance-factor table entries staged in LDS (every inversion draw is below it)
+#define XZ_QCAP 192  // queue entries per wave: two stacks of < 128 entries, fewer than 192 together
+#define XZ_MU_SPLIT 6.0  // proposals with a larger mean go to the high stack
 
+// four waves per SIMD: the register budget of 128 holds without spills
 template <typename T>
-__global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xz(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ uint32_t hist[4][BQ_HLO];
-    __shared__ XzItem queue[4][128];
-    __shared__ double s_spt[XZ_SPT];
+    __shared__ XzItem queue[4][XZ_QCAP];
+    __shared__ double s_odds[XZ_SPT];
     __shared__ double s_inv[BQ_INV_N];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 4 + wv;
     // the grid covers the largest chain of a batch: blocks past this chain's cells leave
     if (blockIdx.x * 4 >= c.N) return;
-    // the softplus table of the batch of the block's first cell; a wave whose cell
+    // the acceptance-factor table of the batch of the block's first cell; a wave whose cell
     // lies in another batch reads its own batch's table from memory
     const int bb = c.batch[blockIdx.x * 4];
-    for (int x = threadIdx.x; x < XZ_SPT; x += 256) s_spt[x] = c.sp_tab[(size_t)bb * H + x];
+    for (int x = threadIdx.x; x < XZ_SPT; x += 256) s_odds[x] = c.odds_tab[(size_t)bb * H + x];
     for (int x = threadIdx.x; x < BQ_INV_N; x += 256) s_inv[x] = bq_inv_tab[x];
     __syncthreads();
     if (i >= c.N) return;
@@ -433,20 +442,60 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
     const double g0 = c.gamma[b];
     const double qz = 1.0 / (1.0 + bq_exp(g0));
     const T edi = ((const T*)c.ed)[i];
-    const T* ea = (const T*)c.ea;
-    const T* eb = (const T*)c.eb + (size_t)k * G;
-    const T* en = (const T*)c.en + (size_t)b * G;
+    const T* tmk = (const T*)c.tm + ((size_t)k * c.B + b) * G;
     auto phi = c.phi + (size_t)b * G;
-    auto spt = c.sp_tab + (size_t)b * H;
+    auto odt = c.odds_tab + (size_t)b * H;
     const bool own = b == bb;
     auto row = c.S + (size_t)i * c.Gp;
     auto zs = c.zs;
     const int z0 = c.zrow[i], z1 = c.zrow[i + 1];
     const uint64_t below = (1ull << lane) - 1ull;
     uint64_t n1 = 0, sx1 = 0;
-    int qn = 0, nt = 0;
+    int nt = 0;
     const int stripe = blockIdx.x % BQ_STRIPES;
     auto tl = c.xz_tail + z0;
+    // Queued proposals: a batch of 64 runs its CDF search up to the largest
+    // draw among its lanes, so proposals are kept in two stacks by their mean:
+    // low means grow from qw[0] up, high means from qw[XZ_QCAP - 1] down.
+    int nlo = 0, nhi = 0;
+    // proposals of the queue entries idx (one per lane < take)
+    auto flush = [&](int take, int idx) {
+        wave_lds_sync();
+        bool tail = false;
+        int tp = 0;
+        if (lane < take) {
+            const XzItem it = qw[idx];
+            const int g = it.g;
+            const int xo = it.s < 0 ? ~it.s : 0;
+            // the stream as bq_xz_update leaves it after the Z draw (three
+            // buffered words) or, without a Z draw, before its first block
+            bq_rng r;
+            r.k0 = c.seed; r.k1 = c.key; r.c0 = (uint32_t)i; r.c1 = (uint32_t)g; r.c2 = (uint32_t)t;
+            r.c3 = ((uint32_t)BQ_PH_XZ << 20) + (xo == 0 ? 1u : 0u);
+            r.b0 = it.b0; r.b1 = it.b1; r.b2 = it.b2; r.b3 = 0u; r.n = xo == 0 ? 3 : 0;
+            const double u = bq_uniform(&r);
+            if (xo < XZ_SPT && bq_nb_inv_ok(it.mu, it.phi)) {
+                // the inversion returns x' <= 255: both acceptance factors are table entries
+                const int xp = bq_rnbinom_inv_t(u, it.mu, it.phi, s_inv);
+                const double eo = own ? s_odds[xo] : odt[xo];
+                const double ep = own ? s_odds[xp] : odt[xp];
+                const int xi = bq_uniform(&r) * ep <= eo ? xp : xo;
+                const int32_t sn = ~xi;
+                if (sn != it.s) { row[g] = sn; zs[it.p] = sn; }
+                n1 += 1;
+                sx1 += (uint64_t)xi;
+                if (xi < BQ_HLO) atomicAdd(&hw[xi], 1u);
+                else gatomic_add(&c.hist_d[b * H + xi], 1u);
+            } else {
+                tail = true;
+                tp = it.p;
+            }
+        }
+        const uint64_t tm = __ballot(tail);
+        if (tail) tl[nt + __popcll(tm & below)] = tp;
+        nt += __popcll(tm);
+        wave_lds_sync();
+    };
     // entries of the current round; the next round's are loaded while this one runs
     int gc = 0;
     int32_t sc = 0;
@@ -462,70 +511,43 @@ __global__ __launch_bounds__(256) void k_xz(const ChainDev* __restrict__ cds)
             const bool act = p < z1;
             int z = 0;
             bq_rng r;
-            T e_a = (T)0, e_b = (T)0, e_n = (T)0;
+            T mc = (T)0;
             double ph = 0.0;
             if (act) {
-                e_a = ea[gc]; e_b = eb[gc]; e_n = en[gc]; ph = phi[gc];
+                mc = tmk[gc]; ph = phi[gc];
                 bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)gc, (uint32_t)t, BQ_PH_XZ);
                 const int xo = sc < 0 ? ~sc : 0;   // X = 0 redraws Z whatever Z was (cu:493-496)
                 z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
             }
-            const uint64_t m = __ballot(act && z);
+            const bool zq = act && z;
+            const T mu = mc * edi;   // the mean of tmu: ((e^alpha e^beta) e^nu) e^delta
+            const bool hq = zq && (double)mu > XZ_MU_SPLIT;
+            const uint64_t mlo = __ballot(zq && !hq), mhi = __ballot(hq);
             const uint64_t m0 = __ballot(act && !z);
-            if (act && z) {
+            if (zq) {
                 XzItem it;
                 it.p = p;
                 it.g = gc;
                 it.s = sc;
                 it.b0 = r.b0; it.b1 = r.b1; it.b2 = r.b2;
-                it.mu = (double)tmu(e_a, e_b, e_n, edi);
+                it.mu = (double)mu;
                 it.phi = ph;
-                qw[qn + __popcll(m & below)] = it;
+                qw[hq ? XZ_QCAP - 1 - (nhi + __popcll(mhi & below)) : nlo + __popcll(mlo & below)] = it;
             }
             if (act && !z && sc != 0) { row[gc] = 0; zs[p] = 0; }
             if (lane == 0 && m0) atomicAdd(&hw[0], (uint32_t)__popcll(m0));
-            qn += __popcll(m);
+            nlo += __popcll(mlo);
+            nhi += __popcll(mhi);
         }
-        if (qn >= 64 || (!more && qn > 0)) {
-            const int take = qn < 64 ? qn : 64;
-            wave_lds_sync();
-            bool tail = false;
-            int tp = 0;
-            if (lane < take) {
-                const XzItem it = qw[qn - take + lane];
-                const int g = it.g;
-                const int xo = it.s < 0 ? ~it.s : 0;
-                // the stream as bq_xz_update leaves it after the Z draw (three
-                // buffered words) or, without a Z draw, before its first block
-                bq_rng r;
-                r.k0 = c.seed; r.k1 = c.key; r.c0 = (uint32_t)i; r.c1 = (uint32_t)g; r.c2 = (uint32_t)t;
-                r.c3 = ((uint32_t)BQ_PH_XZ << 20) + (xo == 0 ? 1u : 0u);
-                r.b0 = it.b0; r.b1 = it.b1; r.b2 = it.b2; r.b3 = 0u; r.n = xo == 0 ? 3 : 0;
-                const double u = bq_uniform(&r);
-                if (xo < XZ_SPT && bq_nb_inv_ok(it.mu, it.phi)) {
-                    // the inversion returns x' <= 255: both softplus terms are table entries
-                    const int xp = bq_rnbinom_inv_t(u, it.mu, it.phi, s_inv);
-                    const double lo = own ? s_spt[xo] : spt[xo];
-                    const double lp = own ? s_spt[xp] : spt[xp];
-                    const int xi = bq_log(bq_uniform(&r)) <= lo - lp ? xp : xo;
-                    const int32_t sn = ~xi;
-                    if (sn != it.s) { row[g] = sn; zs[it.p] = sn; }
-                    n1 += 1;
-                    sx1 += (uint64_t)xi;
-                    if (xi < BQ_HLO) atomicAdd(&hw[xi], 1u);
-                    else gatomic_add(&c.hist_d[b * H + xi], 1u);
-                } else {
-                    tail = true;
-                    tp = it.p;
-                }
-            }
-            const uint64_t tm = __ballot(tail);
-            if (tail) tl[nt + __popcll(tm & below)] = tp;
-            nt += __popcll(tm);
-            qn -= take;
-            wave_lds_sync();
+        // full batches of either stack (the newest 64 entries of it)
+        if (nlo >= 64) { flush(64, nlo - 64 + lane); nlo -= 64; }
+        if (nhi >= 64) { flush(64, XZ_QCAP - nhi + lane); nhi -= 64; }
+        if (!more) {
+            // what remains: one batch when it fits, else the high stack first
+            if (nlo + nhi > 64) { flush(nhi, XZ_QCAP - nhi + lane); nhi = 0; }
+            if (nlo + nhi > 0) flush(nlo + nhi, lane < nlo ? lane : XZ_QCAP - nhi + (lane - nlo));
+            break;
         }
-        if (!more) break;
         gc = gn;
         sc = sn_next;
     }
@@ -1563,6 +1585,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.db = s->dalloc<char>(ts * K * G); c.cb = s->dalloc<char>(ts * K * G);
             c.dn = s->dalloc<char>(ts * B * G); c.cn = s->dalloc<char>(ts * B * G);
             c.pc = s->dalloc<char>(ts * 6 * B * G);
+            c.tm = s->dalloc<char>(ts * K * B * G);
             c.cnt = s->dalloc<int32_t>((size_t)B * K * BQ_STRIPES);
             c.perm = s->dalloc<int32_t>(N);
             c.tiles = s->dalloc<int32_t>((size_t)T_max * 4);
@@ -1576,7 +1599,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.xz_tail = s->dalloc<int32_t>(nz > 0 ? (size_t)nz : 1);
             c.zs = s->dalloc<int32_t>(nz > 0 ? (size_t)nz : 1);   // every Y = 0 entry starts at X = 0, Z = 0
             c.xz_ntail = s->dalloc<int32_t>(N);
-            c.sp_tab = s->dalloc<double>((size_t)B * H);
+            c.odds_tab = s->dalloc<double>((size_t)B * H);
             c.beta_acc = s->dalloc<int32_t>((size_t)K * G);
             c.n_rec = s->dalloc<int>(1);
             c.s_gamma = s->dalloc<double>(2 * (size_t)B); c.s_alpha = s->dalloc<double>(G);
@@ -1612,7 +1635,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             if (rc != BUSSEQ_OK) return fail(rc);
             auto mx = [](int& a, long v) { if (v > a) a = (int)v; };
             mx(s->maxN, N); mx(s->maxG, G); mx(s->maxK, K); mx(s->maxB, B); mx(s->maxT, T_max);
-            mx(s->maxPrep, G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * BQ_HLO * BQ_STRIPES); mx(s->maxPrep, (long)B * K * BQ_STRIPES);
+            mx(s->maxPrep, G); mx(s->maxPrep, (long)K * B * G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * BQ_HLO * BQ_STRIPES); mx(s->maxPrep, (long)B * K * BQ_STRIPES);
             const long nA = (G + 63) / 64;
             mx(s->maxAdec, nA + nblk((long)G * ((K - 1) + (B - 1) + B)));
             mx(s->maxDec, nA * (K - 1)); mx(s->maxDec, nA * (B - 1)); mx(s->maxDec, nA * B);

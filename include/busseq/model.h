@@ -153,18 +153,23 @@ BQ_FN int bq_bern_logodds(double u, double lo)
     return u > 1.0 / (1.0 + bq_exp(lo));
 }
 
-/* softplus(-(g0 + g1*x)): log(1 + e^{-g0-g1 x}); the X MH ratio in log space (cu:504) */
-BQ_FN double bq_neg_logit_sp(double g0, double g1, double x)
+/* 1 + e^{-(g0 + g1 x)}: a factor of the X MH ratio (cu:504).  The exponent is
+ * clamped at 700 so that neither the factor nor u times it overflows; the
+ * clamp changes the ratio only where g0 + g1 x < -700. */
+BQ_FN double bq_xz_odds(double g0, double g1, double x)
 {
-    return bq_softplus(-(g0 + g1 * x));
+    double e = -(g0 + g1 * x);
+    if (e > 700.0) e = 700.0;
+    return 1.0 + bq_exp(e);
 }
 
 /*
  * One X/Z update of an entry with Y = 0 and encoded state s (cu:484-512): Z is
  * redrawn when X = 0 (P(Z = 1) = 1 - qz, qz = 1/(1 + e^{g0})); with Z = 1 a
  * proposal x' ~ NB(mu, phi) is accepted with probability
- * min(1, (1 + e^{-g0-g1 X}) / (1 + e^{-g0-g1 x'})), compared in log space.
- * Returns the new encoded state (0, or ~X with Z = 1).
+ * min(1, (1 + e^{-g0-g1 X}) / (1 + e^{-g0-g1 x'})), tested as
+ * u (1 + e^{-g0-g1 x'}) <= 1 + e^{-g0-g1 X}.  Returns the new encoded state
+ * (0, or ~X with Z = 1).
  */
 BQ_FN int32_t bq_xz_update(int32_t s, double mu, double phi, double g0, double g1, double qz, bq_rng* r)
 {
@@ -172,8 +177,7 @@ BQ_FN int32_t bq_xz_update(int32_t s, double mu, double phi, double g0, double g
     int z = xo == 0 ? (bq_uniform(r) > qz) : 1;
     if (!z) return 0;
     int xp = bq_rnbinom_xz(r, mu, phi);
-    double lr = bq_neg_logit_sp(g0, g1, (double)xo) - bq_neg_logit_sp(g0, g1, (double)xp);
-    int xi = bq_log(bq_uniform(r)) <= lr ? xp : xo;
+    int xi = bq_uniform(r) * bq_xz_odds(g0, g1, (double)xp) <= bq_xz_odds(g0, g1, (double)xo) ? xp : xo;
     return ~xi;
 }
 

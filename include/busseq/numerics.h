@@ -576,19 +576,21 @@ BQ_FN int bq_rnbinom(bq_rng* r, double mu, double phi)
  */
 BQ_FN int bq_nb_inv_ok(double mu, double phi) { return mu <= 30.0 && 3.0 * mu <= 17.0 * phi; }
 
+/* p_k = p_{k-1} c_k with c_k = q ((phi + k - 1) (1/k)); c_{k+1} is formed in
+ * step k, so that its table read overlaps the step (inv needs 1/256 for the
+ * last step). */
 BQ_FN int bq_rnbinom_inv_t(double u, double mu, double phi, const double* inv)
 {
     double q = mu / (phi + mu);
     double p = bq_exp(phi * bq_log1p(-q));
     double s = p;
     int k = 0;
-    double ik = inv[1];   /* 1/k of the next step, read one step ahead */
+    double c = q * (phi * inv[1]);
     while (u > s && k < 255) {
         k += 1;
-        const double ikn = inv[BQ_WAVE_UNIFORM(k + 1)];
-        p *= q * ((phi + (double)(k - 1)) * ik);
+        p *= c;
         s += p;
-        ik = ikn;
+        c = q * ((phi + (double)k) * inv[BQ_WAVE_UNIFORM(k + 1)]);
     }
     return k;
 }
