@@ -13,7 +13,7 @@ import os as _os
 
 import numpy as _np
 
-__all__ = ["Sampler", "simulate", "run", "read_counts", "read_dims", "kernel_names", "copy_probe",
+__all__ = ["Sampler", "simulate", "run", "read_counts", "write_counts_binary", "read_dims", "kernel_names", "copy_probe",
            "autotune", "autotune_batch", "library_path", "BusseqError", "STATUS"]
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
@@ -89,6 +89,7 @@ def _load():
     lib.busseq_result_free.restype = None
     lib.busseq_read_dims.argtypes = [_C.c_char_p, _C.c_int, _C.POINTER(_C.c_int)]
     lib.busseq_read_counts.argtypes = [_C.c_char_p, _C.c_int, _C.c_int, _PI]
+    lib.busseq_write_counts_binary.argtypes = [_C.c_char_p, _C.c_int, _C.c_int, _PI]
     lib.busseq_create_batch.argtypes = [_C.POINTER(_Problem), _C.c_int, _C.POINTER(vp)]
     lib.busseq_chain_count.argtypes = [vp]
     lib.busseq_kernel_count.restype = _C.c_int
@@ -171,6 +172,14 @@ def read_counts(path, N, G):
     Y = _np.zeros((N, G), _np.int32)
     _check(lib.busseq_read_counts(path.encode(), N, G, _ip(Y)))
     return Y
+
+
+def write_counts_binary(path, Y):
+    """Binary form of a count matrix that read_counts (and the executable's -c)
+    reads without parsing text."""
+    lib = _load()
+    Y = _np.ascontiguousarray(Y, dtype=_np.int32)
+    _check(lib.busseq_write_counts_binary(path.encode(), Y.shape[0], Y.shape[1], _ip(Y)))
 
 
 def _result_to_dict(r):

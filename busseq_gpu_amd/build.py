@@ -51,11 +51,13 @@ def build(verbose=True):
             os.path.join(CSRC, "engine.h")]
     eng_o = os.path.join(OBJ, "engine.o")
     host_o = os.path.join(OBJ, "host.o")
+    init_o = os.path.join(OBJ, "init.o")
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "engine.hip"), "-o", eng_o])
+    _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "init.hip"), "-o", init_o])
     cxx = shutil.which("g++") or "g++"
     _run([cxx] + CXX_FLAGS + ["-c", os.path.join(CSRC, "host.cpp"), "-o", host_o])
     so = os.path.join(LIB, "libbusseq.so")
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", so + ".tmp", eng_o, host_o, "-lgomp"])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", so + ".tmp", eng_o, init_o, host_o, "-lgomp"])
     os.replace(so + ".tmp", so)
     cli_o = os.path.join(OBJ, "cli.o")
     _run([cxx, "-O2", "-std=c++17", "-Wall", "-c", os.path.join(CSRC, "cli.cpp"), "-o", cli_o])

@@ -9,7 +9,12 @@
 //   --tile-rows=R|auto  cells per reduction tile (default 64; auto: the fastest of
 //                       16, 32, 64, 128 timed on this data, printed)
 //   --device=D          HIP device ordinal (default 0)
+//   --count-cache=PATH  binary copy of the -c counts: read instead of the text when
+//                       it is at least as new as the -c file and holds N x G counts,
+//                       otherwise written after the text is read
+// The -c file itself may also be a binary count file (busseq_write_counts_binary).
 #include <getopt.h>
+#include <sys/stat.h>
 #include <sys/time.h>
 
 #include <cstdio>
@@ -25,6 +30,33 @@ static int fail(const char* what)
     return 1;
 }
 
+// modification time of a file in nanoseconds, or -1 if it cannot be read
+static long long mtime_ns(const char* path)
+{
+    struct stat st;
+    if (stat(path, &st) != 0) return -1;
+    return (long long)st.st_mtim.tv_sec * 1000000000LL + st.st_mtim.tv_nsec;
+}
+
+// counts from the -c file, or from its binary cache when that is current
+static busseq_status read_counts(const char* count_data, const char* cache, int N, int G, int32_t* Y)
+{
+    if (cache[0]) {
+        const long long tc = mtime_ns(cache), tt = mtime_ns(count_data);
+        if (tc >= 0 && tt >= 0 && tc >= tt && busseq_read_counts(cache, N, G, Y) == BUSSEQ_OK) {
+            std::printf("Counts read from the cache %s.\n", cache);
+            return BUSSEQ_OK;
+        }
+    }
+    busseq_status rc = busseq_read_counts(count_data, N, G, Y);
+    if (rc != BUSSEQ_OK || !cache[0]) return rc;
+    if (busseq_write_counts_binary(cache, N, G, Y) == BUSSEQ_OK)
+        std::printf("Counts cached in %s.\n", cache);
+    else
+        std::printf("Warning: the count cache was not written: %s\n", busseq_last_error());
+    return BUSSEQ_OK;
+}
+
 int main(int argc, char** argv)
 {
     // hard-coded defaults, BUSseq_gpu.cu:1028-1038
@@ -36,6 +68,7 @@ int main(int argc, char** argv)
     int n_iter = 4000, n_burnin = 2000, n_unchanged = 500, print_preserved = 0;
     char output_file[4096] = "demo_output";
     char n_file[4096] = "";
+    char cache[4096] = "";
     int n_file_flag = 0, burnin_flag = 1, unchanged_flag = 1, seed_flag = 1;
     int precision = 32, tile_rows = 64, device = 0, tune = 0;
 
@@ -43,6 +76,7 @@ int main(int argc, char** argv)
         {"precision", required_argument, nullptr, 1000},
         {"tile-rows", required_argument, nullptr, 1001},
         {"device", required_argument, nullptr, 1002},
+        {"count-cache", required_argument, nullptr, 1003},
         {nullptr, 0, nullptr, 0},
     };
     int opt;
@@ -62,6 +96,7 @@ int main(int argc, char** argv)
         case 1000: precision = std::atoi(optarg); break;
         case 1001: tune = std::strcmp(optarg, "auto") == 0; tile_rows = tune ? 64 : std::atoi(optarg); break;
         case 1002: device = std::atoi(optarg); break;
+        case 1003: std::snprintf(cache, sizeof(cache), "%s", optarg); break;
         default:
             std::printf("Error Usage: %s [-b batch] [-n sample_size_file] [-g gene] [-k celltype] [-c count_data_file)] "
                         "[-i iterations] [-b burnin] [-u unchanged_iterations] [-p print_preserved] [-o output_prefix] \n",
@@ -104,7 +139,7 @@ int main(int argc, char** argv)
     std::printf("Start reading file.\n");
     std::fflush(stdout);
     std::vector<int32_t> Y((size_t)N * G);
-    if (busseq_read_counts(count_data, N, G, Y.data()) != BUSSEQ_OK) return fail("reading the count file");
+    if (read_counts(count_data, cache, N, G, Y.data()) != BUSSEQ_OK) return fail("reading the count file");
 
     std::printf("Start initialization.\n");
     std::fflush(stdout);

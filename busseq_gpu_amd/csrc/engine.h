@@ -85,18 +85,29 @@ struct ChainG { BQ_CHAIN_FIELDS(BQ_GLOBAL) };
 static_assert(sizeof(ChainG) == sizeof(ChainDev), "ChainG mirrors ChainDev");
 #endif
 
-// Host-side initial state of one chain (BUSseq_gpu.cu:1163-1244).
+// Host-side initial parameters of one chain (BUSseq_gpu.cu:1163-1244).
 struct HostInit {
-    std::vector<int32_t> S;        // N*Gp
     std::vector<int32_t> W, L;
     std::vector<double> alpha, beta, nu, delta, phi, gamma, pi, ptau;
     std::vector<double> mu_alpha, mu_nu, mu_delta;
-    std::vector<int64_t> hist_s;
-    std::vector<int32_t> zrow;     // CSR of the Y = 0 entries
-    std::vector<uint16_t> zcol;
-    std::vector<int32_t> big_off, big;   // static counts >= the histogram size, per batch
 };
 
-// host.cpp
-int bq_host_init(int B, const int* n_b, int G, int Gp, int K, const int32_t* Y,
-                 uint32_t seed, uint32_t chain, HostInit& out);
+// host.cpp: the closed forms of the initial state.  Stage 1 (pi0, W0, delta0)
+// needs the library sizes; stage 2 (the rest) the exact raw-mean sums rs.
+void bq_init_cells(int B, const int* n_b, int K, const int64_t* lib, uint32_t seed, uint32_t chain, HostInit& o);
+void bq_init_params(int B, const int* n_b, int G, int K, const uint64_t* rs, uint32_t seed, uint32_t chain,
+                    HostInit& o);
+
+// init.hip: the passes over the counts (hipError_t values as int)
+// rows: S (Y > 0 -> Y, else 0, padding 0), per-row counts of zero entries and
+// of entries >= BQ_HIST, library sizes, static histogram hist[b*BQ_HIST + y]
+// of the entries 0 < Y < BQ_HIST (added to, so it starts cleared)
+int bq_dev_init_rows(const int32_t* Yd, int N, int G, int Gp, const int32_t* batch, int32_t* S,
+                     int32_t* nz_row, int32_t* nbig_row, int64_t* lib, int64_t* hist, void* stream);
+// lists: genes of the zero entries of row i at zrow[i], values >= BQ_HIST at brow[i], in gene order
+int bq_dev_init_lists(const int32_t* S, int N, int G, int Gp, const int32_t* zrow, const int32_t* brow,
+                      uint16_t* zcol, int32_t* big, void* stream);
+// raw-mean sums rs[(b*K + k)*G + g] += fix(log1p(S_ig ie_i)) over the cells of each
+// chunk (b, first cell, count) with W_i = k (rs starts cleared)
+int bq_dev_init_raw(const int32_t* S, int G, int Gp, int K, const int32_t* W, const double* ie,
+                    const int32_t* chunks, int n_chunks, uint64_t* rs, void* stream);

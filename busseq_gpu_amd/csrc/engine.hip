@@ -1494,6 +1494,107 @@ struct ProblemDev {
     uint16_t* zcol;
 };
 
+// device buffers that live only while a sampler is created
+struct DevTmp {
+    std::vector<void*> p;
+    bool oom = false;
+    ~DevTmp() { for (void* q : p) (void)hipFree(q); }
+    template <typename V>
+    V* alloc(size_t n)
+    {
+        void* q = nullptr;
+        if (hipMalloc(&q, (n ? n : 1) * sizeof(V)) != hipSuccess) { oom = true; return nullptr; }
+        p.push_back(q);
+        return (V*)q;
+    }
+};
+
+#define INITCHK(x) HIPCHK((hipError_t)(x))
+static const int kRawRows = 64;   // cells per chunk of the raw-mean pass
+
+// Static data of one problem and its initial state matrix S0 (BUSseq_gpu.cu:1188-1189,
+// 1204-1205, 1226-1228), built on the device from one upload of the counts.
+// Returns the library sizes, the number of zero entries and the chunks of the
+// raw-mean pass.
+busseq_status init_problem(busseq_sampler* s, const int32_t* Y, int B, int N, int G, int Gp,
+                           const std::vector<int32_t>& batch, const std::vector<int32_t>& boff, ProblemDev& pd,
+                           int32_t*& S0, std::vector<int64_t>& lib, long& nz, std::vector<int32_t>& chunks)
+{
+    DevTmp tmp;
+    int32_t* Yd = tmp.alloc<int32_t>((size_t)N * G);
+    int32_t* nzr = tmp.alloc<int32_t>(N);
+    int32_t* nbr = tmp.alloc<int32_t>(N);
+    int64_t* libd = tmp.alloc<int64_t>(N);
+    int32_t* browd = tmp.alloc<int32_t>((size_t)N + 1);
+    pd.batch = s->dalloc<int32_t>(N); pd.boff = s->dalloc<int32_t>(B + 1);
+    pd.hist = s->dalloc<int64_t>((size_t)B * H); pd.zrow = s->dalloc<int32_t>((size_t)N + 1);
+    pd.big_off = s->dalloc<int32_t>((size_t)B + 1);
+    S0 = s->dalloc<int32_t>((size_t)N * Gp);
+    if (tmp.oom || s->oom) { bq_set_error("device allocation failed"); return BUSSEQ_ERR_NOMEM; }
+    HIPCHK(hipDeviceSynchronize());   // the clears of dalloc precede the passes on the sampler's stream
+    HIPCHK(hipMemcpy(Yd, Y, (size_t)N * G * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(h2d(pd.batch, batch));
+    HIPCHK(h2d(pd.boff, boff));
+    INITCHK(bq_dev_init_rows(Yd, N, G, Gp, pd.batch, S0, nzr, nbr, libd, pd.hist, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    std::vector<int32_t> nzh, nbh;
+    HIPCHK(d2h(nzh, nzr, N));
+    HIPCHK(d2h(nbh, nbr, N));
+    HIPCHK(d2h(lib, libd, N));
+    for (int i = 0; i < N; ++i)
+        if (lib[i] <= 0) {
+            bq_set_error("cell " + std::to_string(i) + " has no counts (zero library size)");
+            return BUSSEQ_ERR_DATA;
+        }
+    std::vector<int32_t> zrow((size_t)N + 1, 0), brow((size_t)N + 1, 0), big_off((size_t)B + 1, 0);
+    for (int i = 0; i < N; ++i) {
+        zrow[(size_t)i + 1] = zrow[i] + nzh[i];
+        brow[(size_t)i + 1] = brow[i] + nbh[i];
+    }
+    for (int b = 0; b < B; ++b) big_off[(size_t)b + 1] = brow[boff[b + 1]];
+    nz = zrow[N];
+    pd.zcol = s->dalloc<uint16_t>((size_t)nz);
+    pd.big = s->dalloc<int32_t>((size_t)brow[N]);
+    if (s->oom) { bq_set_error("device allocation failed"); return BUSSEQ_ERR_NOMEM; }
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(h2d(pd.zrow, zrow));
+    HIPCHK(h2d(pd.big_off, big_off));
+    HIPCHK(hipMemcpy(browd, brow.data(), brow.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    INITCHK(bq_dev_init_lists(S0, N, G, Gp, pd.zrow, browd, pd.zcol, pd.big, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    chunks.clear();
+    for (int b = 0; b < B; ++b)
+        for (int st = boff[b]; st < boff[b + 1]; st += kRawRows) {
+            chunks.push_back(b);
+            chunks.push_back(st);
+            chunks.push_back(boff[b + 1] - st < kRawRows ? boff[b + 1] - st : kRawRows);
+        }
+    return BUSSEQ_OK;
+}
+
+// Raw-mean sums of one chain (cu:369-382, 1210-1221) from its W0 (already in
+// Wd) and delta0: rs[(b*K + k)*G + g], exact fixed point.
+busseq_status init_raw_sums(busseq_sampler* s, const int32_t* S0, int B, int N, int G, int Gp, int K,
+                            const int32_t* Wd, const std::vector<double>& delta, const std::vector<int32_t>& chunks,
+                            std::vector<uint64_t>& rs)
+{
+    DevTmp tmp;
+    const size_t nrs = (size_t)B * K * G;
+    uint64_t* rsd = tmp.alloc<uint64_t>(nrs);
+    double* ied = tmp.alloc<double>(N);
+    int32_t* chd = tmp.alloc<int32_t>(chunks.size());
+    if (tmp.oom) { bq_set_error("device allocation failed"); return BUSSEQ_ERR_NOMEM; }
+    std::vector<double> ie(N);
+    for (int i = 0; i < N; ++i) ie[i] = bq_exp(-delta[i]);
+    HIPCHK(h2d(ied, ie));
+    HIPCHK(h2d(chd, chunks));
+    HIPCHK(hipMemsetAsync(rsd, 0, nrs * sizeof(uint64_t), s->stream));
+    INITCHK(bq_dev_init_raw(S0, G, Gp, K, Wd, ied, chd, (int)(chunks.size() / 3), rsd, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(d2h(rs, rsd, nrs));
+    return BUSSEQ_OK;
+}
+
 } // namespace
 
 extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_probs, busseq_sampler** out)
@@ -1536,28 +1637,18 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
         for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + n_b[b];
         for (int b = 0; b < B; ++b) for (int i = boff[b]; i < boff[b + 1]; ++i) batch[i] = b;
         ProblemDev pd{};
+        int32_t* S0 = nullptr;   // initial state matrix: chain 0's S, copied to the others
+        std::vector<int64_t> lib;
+        std::vector<int32_t> chunks;
+        long nz = 0;             // entries with Y = 0
+        {
+            busseq_status rc = init_problem(s, Y, B, N, G, Gp, batch, boff, pd, S0, lib, nz, chunks);
+            if (rc != BUSSEQ_OK) return fail(rc);
+        }
         for (int k = 0; k < cfg->chains; ++k) {
             HostInit hi;
             uint32_t key = (uint32_t)(cfg->chain0 + k);
-            if (bq_host_init(B, n_b.data(), G, Gp, K, Y, cfg->seed, key, hi) != 0) return fail(BUSSEQ_ERR_DATA);
-            const long nz = (long)hi.zcol.size();   // entries with Y = 0
-            if (k == 0) {
-                pd.batch = s->dalloc<int32_t>(N); pd.boff = s->dalloc<int32_t>(B + 1);
-                pd.hist = s->dalloc<int64_t>((size_t)B * H); pd.zrow = s->dalloc<int32_t>((size_t)N + 1);
-                pd.big_off = s->dalloc<int32_t>((size_t)B + 1); pd.zcol = s->dalloc<uint16_t>(hi.zcol.size());
-                pd.big = s->dalloc<int32_t>(hi.big.size());
-                if (!pd.batch || !pd.boff || !pd.hist || !pd.zrow || !pd.big_off || !pd.zcol || !pd.big) {
-                    bq_set_error("device allocation failed");
-                    return fail(BUSSEQ_ERR_NOMEM);
-                }
-                if (h2d(pd.batch, batch) != hipSuccess || h2d(pd.boff, boff) != hipSuccess ||
-                    h2d(pd.hist, hi.hist_s) != hipSuccess || h2d(pd.zrow, hi.zrow) != hipSuccess ||
-                    h2d(pd.big_off, hi.big_off) != hipSuccess || h2d(pd.zcol, hi.zcol) != hipSuccess ||
-                    h2d(pd.big, hi.big) != hipSuccess) {
-                    bq_set_error("upload failed");
-                    return fail(BUSSEQ_ERR_DEVICE);
-                }
-            }
+            bq_init_cells(B, n_b.data(), K, lib.data(), cfg->seed, key, hi);
             ChainDev c{};
             c.N = N; c.G = G; c.Gp = Gp; c.K = K; c.B = B;
             c.seed = cfg->seed; c.key = key;
@@ -1567,7 +1658,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.batch = pd.batch; c.boff = pd.boff; c.hist_s = pd.hist;
             c.zrow = pd.zrow; c.zcol = pd.zcol; c.big_off = pd.big_off; c.big = pd.big;
             c.iter = s->dalloc<int>(1);
-            c.S = s->dalloc<int32_t>((size_t)N * Gp);
+            c.S = k == 0 ? S0 : s->dalloc<int32_t>((size_t)N * Gp);
             c.W = s->dalloc<int32_t>(N);
             c.L = s->dalloc<int32_t>((size_t)K * G);
             c.alpha = s->dalloc<double>(G); c.beta = s->dalloc<double>((size_t)K * G);
@@ -1619,7 +1710,19 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
                 bq_set_error("device allocation failed");
                 return fail(BUSSEQ_ERR_NOMEM);
             }
-            if (h2d(c.S, hi.S) != hipSuccess || h2d(c.W, hi.W) != hipSuccess || h2d(c.L, hi.L) != hipSuccess ||
+            if (hipDeviceSynchronize() != hipSuccess ||
+                (k > 0 && hipMemcpy(c.S, S0, (size_t)N * Gp * sizeof(int32_t), hipMemcpyDeviceToDevice) != hipSuccess) ||
+                h2d(c.W, hi.W) != hipSuccess) {
+                bq_set_error("upload of the initial state failed");
+                return fail(BUSSEQ_ERR_DEVICE);
+            }
+            {
+                std::vector<uint64_t> rs;
+                busseq_status rc = init_raw_sums(s, S0, B, N, G, Gp, K, c.W, hi.delta, chunks, rs);
+                if (rc != BUSSEQ_OK) return fail(rc);
+                bq_init_params(B, n_b.data(), G, K, rs.data(), cfg->seed, key, hi);
+            }
+            if (h2d(c.L, hi.L) != hipSuccess ||
                 h2d(c.alpha, hi.alpha) != hipSuccess || h2d(c.beta, hi.beta) != hipSuccess ||
                 h2d(c.nu, hi.nu) != hipSuccess || h2d(c.delta, hi.delta) != hipSuccess ||
                 h2d(c.phi, hi.phi) != hipSuccess || h2d(c.gamma, hi.gamma) != hipSuccess ||

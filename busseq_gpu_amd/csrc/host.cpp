@@ -3,7 +3,13 @@
 // the synthetic-data generator and error reporting.  Compiled with g++
 // (-ffp-contract=off) so that the initial state uses the same numerics as the
 // device code.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <cerrno>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -33,45 +39,18 @@ extern "C" void busseq_config_default(busseq_config* c)
 }
 
 // ---------------------------------------------------------------------------
-// initial state, BUSseq_gpu.cu:1163-1244
+// initial state, BUSseq_gpu.cu:1163-1244.  The passes over the N x G counts
+// (state matrix, zero-entry lists, library sizes, raw-mean sums) run on the
+// device (init.hip); the closed forms over cells and over (batch, subtype,
+// gene) run here, in two stages around the raw-mean pass.
 // ---------------------------------------------------------------------------
-int bq_host_init(int B, const int* n_b, int G, int Gp, int K, const int32_t* Y,
-                 uint32_t seed, uint32_t chain, HostInit& o)
+
+// stage 1: pi0, W0 and delta0 of one chain from the library sizes
+void bq_init_cells(int B, const int* n_b, int K, const int64_t* lib, uint32_t seed, uint32_t chain, HostInit& o)
 {
-    const int H = BQ_HIST;
-    std::vector<int> boff(B + 1, 0), batch;
+    std::vector<int> boff(B + 1, 0);
     for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + n_b[b];
     const int N = boff[B];
-    batch.resize(N);
-    for (int b = 0; b < B; ++b)
-        for (int i = boff[b]; i < boff[b + 1]; ++i) batch[i] = b;
-
-    o.S.assign((size_t)N * Gp, 0);
-    o.hist_s.assign((size_t)B * H, 0);
-    o.zrow.assign((size_t)N + 1, 0);
-    o.zcol.clear();
-    o.big_off.assign((size_t)B + 1, 0);
-    o.big.clear();
-    std::vector<int64_t> lib(N, 0);
-    for (int i = 0; i < N; ++i) {
-        int64_t s = 0;
-        for (int g = 0; g < G; ++g) {
-            int32_t y = Y[(size_t)i * G + g];
-            int32_t v = y > 0 ? y : 0;
-            o.S[(size_t)i * Gp + g] = v;
-            s += v;
-            if (v > 0 && v < H) o.hist_s[(size_t)batch[i] * H + v] += 1;
-            if (v == 0) o.zcol.push_back((uint16_t)g);
-            if (v >= H) { o.big.push_back(v); o.big_off[(size_t)batch[i] + 1] += 1; }
-        }
-        o.zrow[(size_t)i + 1] = (int32_t)o.zcol.size();
-        lib[i] = s;
-        if (s <= 0) {
-            bq_set_error("cell " + std::to_string(i) + " has no counts (zero library size)");
-            return -1;
-        }
-    }
-    for (int b = 0; b < B; ++b) o.big_off[(size_t)b + 1] += o.big_off[b];
     // pi0_bk = (k+1)/(K(K+1)/2)  (cu:392-397)
     o.pi.assign((size_t)B * K, 0.0);
     for (int b = 0; b < B; ++b)
@@ -79,30 +58,31 @@ int bq_host_init(int B, const int* n_b, int G, int Gp, int K, const int32_t* Y,
     // W0 ~ Cat(pi0_b); delta0_i = log s_i - log s_first(b)  (cu:419-443)
     o.W.assign(N, 0);
     o.delta.assign(N, 0.0);
-    for (int i = 0; i < N; ++i) {
-        int b = batch[i];
-        bq_rng r;
-        bq_rng_init(&r, seed, chain, (uint32_t)i, 0u, BQ_SWEEP_INIT, BQ_PH_INIT_W);
-        double u = bq_uniform(&r);
-        int k = 0;
-        double tmp = o.pi[(size_t)b * K];
-        while (u > tmp && k < K - 1) { k++; tmp += o.pi[(size_t)b * K + k]; }
-        o.W[i] = k;
-        o.delta[i] = bq_log((double)lib[i]) - bq_log((double)lib[boff[b]]);
-    }
-    // raw means r_bkg (cu:369-382, 1210-1221) as exact sums of log(1 + Y e^-delta)
-    std::vector<uint64_t> rs((size_t)B * K * G, 0);
-    std::vector<int> nbk((size_t)B * K, 0);
-    for (int i = 0; i < N; ++i) {
-        int b = batch[i], k = o.W[i];
-        nbk[(size_t)b * K + k] += 1;
-        double ie = bq_exp(-o.delta[i]);
-        uint64_t* row = &rs[((size_t)b * K + k) * G];
-        for (int g = 0; g < G; ++g) {
-            int32_t y = o.S[(size_t)i * Gp + g];
-            row[g] += bq_fix(bq_log1p((double)y * ie));
+    for (int b = 0; b < B; ++b)
+        for (int i = boff[b]; i < boff[b + 1]; ++i) {
+            bq_rng r;
+            bq_rng_init(&r, seed, chain, (uint32_t)i, 0u, BQ_SWEEP_INIT, BQ_PH_INIT_W);
+            double u = bq_uniform(&r);
+            int k = 0;
+            double tmp = o.pi[(size_t)b * K];
+            while (u > tmp && k < K - 1) { k++; tmp += o.pi[(size_t)b * K + k]; }
+            o.W[i] = k;
+            o.delta[i] = bq_log((double)lib[i]) - bq_log((double)lib[boff[b]]);
         }
-    }
+    o.mu_delta = o.delta;
+}
+
+// stage 2: the remaining parameters from the exact sums
+// rs[(b*K + k)*G + g] = sum over cells i of batch b with W0_i = k of fix(log1p(Y_ig e^-delta0_i))
+void bq_init_params(int B, const int* n_b, int G, int K, const uint64_t* rs, uint32_t seed, uint32_t chain,
+                    HostInit& o)
+{
+    const int N = (int)o.W.size();
+    std::vector<int> boff(B + 1, 0), nbk((size_t)B * K, 0);
+    for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + n_b[b];
+    for (int b = 0; b < B; ++b)
+        for (int i = boff[b]; i < boff[b + 1] && i < N; ++i) nbk[(size_t)b * K + o.W[i]] += 1;
+    // raw means r_bkg (cu:369-382, 1210-1221)
     std::vector<double> rm((size_t)B * K * G, 0.0);
     for (int b = 0; b < B; ++b)
         for (int k = 0; k < K; ++k)
@@ -162,8 +142,6 @@ int bq_host_init(int B, const int* n_b, int G, int Gp, int K, const int32_t* Y,
             bq_rng_init(&r, seed, chain, (uint32_t)g, (uint32_t)k, BQ_SWEEP_INIT, BQ_PH_INIT_L);
             o.L[j] = bq_bern_logodds(bq_uniform(&r), lr);
         }
-    o.mu_delta = o.delta;
-    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -226,23 +204,175 @@ extern "C" busseq_status busseq_read_dims(const char* path, int B, int* n_b)
     return BUSSEQ_OK;
 }
 
-extern "C" busseq_status busseq_read_counts(const char* path, int N, int G, int32_t* Y)
+static inline bool is_ws(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\f' || c == '\v'; }
+
+// Binary count file: the 32-byte header below, then N*G little-endian int32,
+// cells as rows.  Written by busseq_write_counts_binary; busseq_read_counts
+// recognises it by the magic, so either form can be given to -c.
+static const char kBinMagic[8] = {'B', 'Q', 'C', 'O', 'U', 'N', 'T', '1'};
+struct BinHeader {
+    char magic[8];
+    int64_t N, G, reserved;
+};
+
+struct MappedFile {
+    const char* p = nullptr;
+    size_t n = 0;
+    int fd = -1;
+    ~MappedFile()
+    {
+        if (p && n) munmap((void*)p, n);
+        if (fd >= 0) close(fd);
+    }
+    bool open_ro(const char* path)
+    {
+        fd = ::open(path, O_RDONLY);
+        if (fd < 0) { bq_set_error(std::string("cannot open ") + path + ": " + std::strerror(errno)); return false; }
+        struct stat st;
+        if (fstat(fd, &st) != 0) { bq_set_error(std::string("cannot size ") + path); return false; }
+        n = (size_t)st.st_size;
+        if (n == 0) return true;
+        void* q = mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (q == MAP_FAILED) { bq_set_error(std::string("cannot map ") + path + ": " + std::strerror(errno)); n = 0; return false; }
+        p = (const char*)q;
+        madvise(q, n, MADV_SEQUENTIAL);
+        return true;
+    }
+};
+
+// Text counts (cu:1145-1159), parsed by T threads: each thread takes the tokens
+// that start in its byte range (ranges end at whitespace, so no token is cut),
+// counts them, and after a prefix sum over the threads parses them into place.
+// Errors are those of a serial reader: the first bad token among the first
+// N*G, else a short file, else the first negative count.
+static busseq_status read_counts_text(const char* path, const char* buf, size_t n, long want, int G, int32_t* Y)
 {
-    std::vector<char> buf;
-    if (!read_file(path, buf)) return BUSSEQ_ERR_IO;
-    long want = (long)N * G;
-    std::string bad;
-    long n = parse_ints(buf.data(), buf.data() + buf.size() - 1, want, Y, bad);
-    if (n < 0) { bq_set_error(std::string(path) + ": not an integer count: '" + bad + "'"); return BUSSEQ_ERR_FORMAT; }
-    if (n < want) {
-        bq_set_error(std::string(path) + ": expected " + std::to_string(want) + " counts (N x G), found " + std::to_string(n));
+    int T = 1;
+#ifdef _OPENMP
+    T = omp_get_max_threads();
+#endif
+    const size_t per = (size_t)4 << 20;   // at least 4 MB per thread
+    if ((size_t)T > n / per + 1) T = (int)(n / per + 1);
+    if (T < 1) T = 1;
+    std::vector<size_t> lo(T + 1, n);
+    lo[0] = 0;
+    for (int t = 1; t < T; ++t) {
+        size_t q = n / T * t;
+        while (q < n && !is_ws(buf[q])) ++q;
+        lo[t] = q;
+    }
+    for (int t = 1; t <= T; ++t) if (lo[t] < lo[t - 1]) lo[t] = lo[t - 1];
+    std::vector<long> cnt(T + 1, 0);
+    #pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; ++t) {
+        long c = 0;
+        char prev = lo[t] == 0 ? ' ' : buf[lo[t] - 1];
+        for (size_t q = lo[t]; q < lo[t + 1]; ++q) {
+            const char ch = buf[q];
+            c += is_ws(prev) && !is_ws(ch);
+            prev = ch;
+        }
+        cnt[t + 1] = c;
+    }
+    for (int t = 0; t < T; ++t) cnt[t + 1] += cnt[t];
+    std::vector<long> bad_at(T, -1), neg_at(T, -1);
+    std::vector<std::string> bad_tok(T);
+    #pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; ++t) {
+        long j = cnt[t];
+        const char* p = buf + lo[t];
+        const char* end = buf + lo[t + 1];
+        while (j < want) {
+            while (p < end && is_ws(*p)) ++p;
+            if (p >= end) break;
+            const char* tok = p;
+            bool neg = false;
+            if (*p == '+' || *p == '-') { neg = *p == '-'; ++p; }
+            int64_t v = 0;
+            const char* d0 = p;
+            while (p < end && *p >= '0' && *p <= '9') { v = v * 10 + (*p - '0'); if (v > 2147483647LL) break; ++p; }
+            if (p == d0 || !(p >= end || is_ws(*p)) || v > 2147483647LL) {
+                const char* q = tok;
+                while (q < end && q - tok < 24 && !is_ws(*q)) ++q;
+                bad_at[t] = j;
+                bad_tok[t].assign(tok, q);
+                break;
+            }
+            const int32_t y = (int32_t)(neg ? -v : v);
+            if (y < 0 && neg_at[t] < 0) neg_at[t] = j;
+            Y[j++] = y;
+        }
+    }
+    for (int t = 0; t < T; ++t)
+        if (bad_at[t] >= 0) {
+            bq_set_error(std::string(path) + ": not an integer count: '" + bad_tok[t] + "'");
+            return BUSSEQ_ERR_FORMAT;
+        }
+    if (cnt[T] < want) {
+        bq_set_error(std::string(path) + ": expected " + std::to_string(want) + " counts (N x G), found " + std::to_string(cnt[T]));
         return BUSSEQ_ERR_FORMAT;
     }
-    for (long j = 0; j < want; ++j)
-        if (Y[j] < 0) {
+    for (int t = 0; t < T; ++t)
+        if (neg_at[t] >= 0) {
+            const long j = neg_at[t];
             bq_set_error(std::string(path) + ": negative count at cell " + std::to_string(j / G) + ", gene " + std::to_string(j % G));
             return BUSSEQ_ERR_FORMAT;
         }
+    return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_read_counts(const char* path, int N, int G, int32_t* Y)
+{
+    if (!path || !Y || N < 0 || G < 0) { bq_set_error("bad arguments to busseq_read_counts"); return BUSSEQ_ERR_ARG; }
+    MappedFile f;
+    if (!f.open_ro(path)) return BUSSEQ_ERR_IO;
+    const long want = (long)N * G;
+    if (f.n >= sizeof(BinHeader) && std::memcmp(f.p, kBinMagic, sizeof(kBinMagic)) == 0) {
+        BinHeader h;
+        std::memcpy(&h, f.p, sizeof(h));
+        if (h.N != N || h.G != G) {
+            bq_set_error(std::string(path) + ": binary count file holds " + std::to_string(h.N) + " x " +
+                         std::to_string(h.G) + " counts, expected " + std::to_string(N) + " x " + std::to_string(G));
+            return BUSSEQ_ERR_FORMAT;
+        }
+        if (f.n != sizeof(BinHeader) + (size_t)want * sizeof(int32_t)) {
+            bq_set_error(std::string(path) + ": binary count file has the wrong size");
+            return BUSSEQ_ERR_FORMAT;
+        }
+        const int32_t* src = (const int32_t*)(f.p + sizeof(BinHeader));
+        long first_neg = LONG_MAX;
+        #pragma omp parallel for schedule(static) reduction(min : first_neg)
+        for (long r = 0; r < N; ++r) {
+            std::memcpy(Y + r * G, src + r * G, (size_t)G * sizeof(int32_t));
+            for (long g = 0; g < G; ++g)
+                if (src[r * G + g] < 0) { if (r * G + g < first_neg) first_neg = r * G + g; break; }
+        }
+        if (first_neg != LONG_MAX) {
+            bq_set_error(std::string(path) + ": negative count at cell " + std::to_string(first_neg / G) + ", gene " + std::to_string(first_neg % G));
+            return BUSSEQ_ERR_FORMAT;
+        }
+        return BUSSEQ_OK;
+    }
+    return read_counts_text(path, f.p ? f.p : "", f.n, want, G, Y);
+}
+
+extern "C" busseq_status busseq_write_counts_binary(const char* path, int N, int G, const int32_t* Y)
+{
+    if (!path || !Y || N < 0 || G < 0) { bq_set_error("bad arguments to busseq_write_counts_binary"); return BUSSEQ_ERR_ARG; }
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) { bq_set_error(std::string("cannot create ") + tmp + ": " + std::strerror(errno)); return BUSSEQ_ERR_IO; }
+    BinHeader h;
+    std::memcpy(h.magic, kBinMagic, sizeof(kBinMagic));
+    h.N = N; h.G = G; h.reserved = 0;
+    const size_t n = (size_t)N * G;
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && (n == 0 || std::fwrite(Y, sizeof(int32_t), n, f) == n);
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        bq_set_error(std::string("cannot write ") + path + ": " + std::strerror(errno));
+        return BUSSEQ_ERR_IO;
+    }
     return BUSSEQ_OK;
 }
 

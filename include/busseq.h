@@ -119,7 +119,15 @@ int busseq_chain_count(const busseq_sampler* s);
 /* Input files.  -N file: B whitespace-separated ints (cu:1094-1098).
  * -c file: N*G whitespace-separated ints, cells as rows (cu:1145-1159). */
 busseq_status busseq_read_dims(const char* path, int B, int* n_b);
+/* Replaces the fscanf loop of cu:1145-1159.  Text is parsed by all OpenMP
+ * threads; the error reported is the one a serial reader meets first (a bad
+ * token among the first N*G, a short file, a negative count).  A file written
+ * by busseq_write_counts_binary is recognised by its header and read as is. */
 busseq_status busseq_read_counts(const char* path, int N, int G, int32_t* Y);
+/* Binary count file (no counterpart in the reference): a 32-byte header
+ * ("BQCOUNT1", int64 N, int64 G, int64 0) and N*G little-endian int32, cells
+ * as rows; written to path.tmp and renamed, so readers never see a partial file. */
+busseq_status busseq_write_counts_binary(const char* path, int N, int G, const int32_t* Y);
 
 /* Synthetic counts from the BUSseq generative model (no counterpart in the
  * reference): alpha_g ~ N(1.5, 0.5^2); beta_gk = 0 for k = 0 and N(+-2, 0.5^2)

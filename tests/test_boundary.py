@@ -83,6 +83,77 @@ def test_read_counts_roundtrip_and_errors(tmp_path):
     assert e.value.status == 3
 
 
+def _text(Y, rng):
+    # every token followed by a random run of mixed whitespace (CRLF, tabs, blanks)
+    seps = np.array([" ", "  ", "\t", "\n", "\r\n", " \t "])
+    pick = seps[rng.integers(0, len(seps), size=Y.size)]
+    return "".join(f"{v}{s}" for v, s in zip(Y.ravel().tolist(), pick.tolist()))
+
+
+def test_read_counts_parallel_text_matches_serial_semantics(tmp_path):
+    # about 16 MB of text: several reader threads, token boundaries at random offsets
+    rng = np.random.default_rng(4)
+    N, G = 2500, 1200
+    Y = rng.negative_binomial(2, 0.2, size=(N, G)).astype(np.int32)
+    Y[rng.integers(0, N, 50), rng.integers(0, G, 50)] = 2147483647
+    body = _text(Y, rng)
+    p = tmp_path / "big.txt"
+    p.write_text(body + " 7 8 junk")                            # extra tokens after N*G are ignored
+    np.testing.assert_array_equal(bq.read_counts(str(p), N, G), Y)
+    toks = body.split()
+    late = N * G - 1000                                          # in the last thread's range
+    bad = list(toks)
+    bad[late] = "12x"
+    bad[late - 900000] = "-4"                                    # a negative before the bad token
+    p.write_text(" ".join(bad))
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), N, G)
+    assert e.value.status == 3 and "'12x'" in str(e.value)      # a serial reader stops at the bad token
+    bad[late] = "1"
+    p.write_text(" ".join(bad))
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), N, G)
+    j = late - 900000
+    assert e.value.status == 3 and f"cell {j // G}, gene {j % G}" in str(e.value)
+    p.write_text(" ".join(toks[:-3]))
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), N, G)
+    assert f"found {N * G - 3}" in str(e.value)
+    p.write_text(" ".join(toks + ["2147483648"]))               # overflow after N*G: ignored
+    np.testing.assert_array_equal(bq.read_counts(str(p), N, G), Y)
+    p.write_text(" ".join(toks[:-1] + ["2147483648"]))          # overflow within N*G
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), N, G)
+    assert "2147483648" in str(e.value)
+
+
+def test_binary_counts_roundtrip_and_errors(tmp_path):
+    rng = np.random.default_rng(5)
+    Y = rng.integers(0, 1000, size=(37, 11)).astype(np.int32)
+    p = tmp_path / "c.bqc"
+    bq.write_counts_binary(str(p), Y)
+    raw = p.read_bytes()
+    assert raw[:8] == b"BQCOUNT1" and len(raw) == 32 + 4 * Y.size
+    assert np.frombuffer(raw[8:32], "<i8").tolist() == [37, 11, 0]
+    np.testing.assert_array_equal(np.frombuffer(raw[32:], "<i4").reshape(37, 11), Y)
+    np.testing.assert_array_equal(bq.read_counts(str(p), 37, 11), Y)
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), 11, 37)
+    assert e.value.status == 3 and "37 x 11" in str(e.value)
+    p.write_bytes(raw[:-4])
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), 37, 11)
+    assert e.value.status == 3 and "size" in str(e.value)
+    Y[20, 3] = -2
+    bq.write_counts_binary(str(p), Y)
+    with pytest.raises(bq.BusseqError) as e:
+        bq.read_counts(str(p), 37, 11)
+    assert "cell 20, gene 3" in str(e.value)
+    with pytest.raises(bq.BusseqError) as e:
+        bq.write_counts_binary(str(tmp_path / "no" / "dir.bqc"), Y)
+    assert e.value.status == 2
+
+
 def fmt(v):
     return "%f" % v
 
@@ -188,3 +259,29 @@ def test_cli_derived_defaults(tmp_path):
     # without -b, burn-in = n_iter/2 (cu:1101-1103): -i 10 gives burn-in 5 < 10, valid until the count file
     p = cli("-B", "2", "-s", "4", "-i", "10", "-c", str(tmp_path / "none.txt"))
     assert "Burn-in" not in p.stdout and p.returncode == 1
+
+
+def test_cli_count_cache(tmp_path):
+    # the cache is written after the text is read, read back while it is current and
+    # ignored once the text is newer; the run itself stops at initialization without a GPU
+    rng = np.random.default_rng(6)
+    n_b = [6, 4]
+    Y = rng.integers(1, 30, size=(10, 8)).astype(np.int32)
+    dims = tmp_path / "dim.txt"
+    dims.write_text("6 4\n")
+    txt = tmp_path / "c.txt"
+    txt.write_text("\n".join(" ".join(map(str, r)) for r in Y) + "\n")
+    cache = tmp_path / "c.bqc"
+    args = ("-B", "2", "-N", str(dims), "-G", "8", "-K", "2", "-s", "1", "-c", str(txt), "-i", "4", "-b", "2",
+            "-o", str(tmp_path / "o"), f"--count-cache={cache}")
+    p = cli(*args)
+    assert f"Counts cached in {cache}." in p.stdout
+    np.testing.assert_array_equal(bq.read_counts(str(cache), 10, 8), Y)
+    p = cli(*args)
+    assert f"Counts read from the cache {cache}." in p.stdout
+    os.utime(txt, ns=(os.stat(cache).st_mtime_ns + 10**9,) * 2)
+    p = cli(*args)
+    assert "read from the cache" not in p.stdout and "Counts cached in" in p.stdout
+    # a binary count file is accepted as -c directly
+    p = cli(*args[:11], str(cache), *args[12:-1])
+    assert "Start initialization." in p.stdout

@@ -120,3 +120,21 @@ def test_c1_shape_parity():
     Y = bq.simulate(n_b, 3000, 5, 1001)
     s, o = run_pair(Y, n_b, 5, (1, 2), 32, n_iter=4000, n_burnin=2000, n_unchanged=500)
     s.close()
+
+
+def test_cell_without_counts_is_rejected():
+    # the device init pass reports a zero library size (the reference would divide by it)
+    n_b = [6, 6]
+    Y = bq.simulate(n_b, 40, 2, 3)
+    Y[7, :] = 0
+    with pytest.raises(bq.BusseqError) as e:
+        bq.Sampler(Y, n_b, 2, n_iter=4, n_burnin=2, n_unchanged=1)
+    assert e.value.status == 4 and "cell 7" in str(e.value)
+
+
+def test_raw_means_span_batches_and_gene_slabs():
+    # several 64-cell chunks per batch, batch ends inside a chunk, G not a multiple of 256
+    n_b = [150, 70, 131]
+    Y = bq.simulate(n_b, 300, 3, 17)
+    s, o = run_pair(Y, n_b, 3, (1,), 32, tile_rows=16)
+    s.close()
