@@ -8,8 +8,10 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cerrno>
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -448,72 +450,109 @@ extern "C" busseq_status busseq_simulate(int B, const int* n_b, int G, int K, ui
 // output writers, byte-compatible with BUSseq_gpu.cu:1610-2018
 // ---------------------------------------------------------------------------
 namespace {
-struct Out {
-    FILE* f = nullptr;
-    std::vector<char> buf;
-    bool ok = true;
-    explicit Out(const std::string& path)
-    {
-        f = std::fopen(path.c_str(), "w");
-        if (!f) { ok = false; bq_set_error("cannot write " + path + ": " + std::strerror(errno)); }
-        buf.reserve(1 << 20);
-    }
-    void flush()
-    {
-        if (f && !buf.empty()) { if (std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) ok = false; buf.clear(); }
-    }
-    void put(const char* s, size_t n) { buf.insert(buf.end(), s, s + n); if (buf.size() > (1u << 20)) flush(); }
-    void i32(int32_t v, char sep)
-    {
-        char t[16]; int n = 0;
-        uint32_t u = v < 0 ? (uint32_t)(-(int64_t)v) : (uint32_t)v;
-        char d[12]; int m = 0;
-        do { d[m++] = (char)('0' + u % 10); u /= 10; } while (u);
-        if (v < 0) t[n++] = '-';
-        while (m) t[n++] = d[--m];
-        t[n++] = sep;
-        put(t, (size_t)n);
-    }
-    void f64(double v, char sep)
-    {
-        char t[400];
-        int n = std::snprintf(t, sizeof(t), "%lf", v);
-        t[n++] = sep;
-        put(t, (size_t)n);
-    }
-    bool close()
-    {
-        flush();
-        if (f && std::fclose(f) != 0) ok = false;
-        f = nullptr;
-        return ok;
-    }
-    ~Out() { if (f) std::fclose(f); }
-};
 
-// rows x cols matrix, tab-separated, newline per row
-template <typename V>
-bool write_matrix(const std::string& path, const V* a, long rows, long cols)
+// "%lf" of v: six decimals of the exact binary value rounded half to even,
+// as glibc's printf does, from 128-bit integer arithmetic (|v| < 9e12; other
+// values go through snprintf).  Returns the length written to t (>= 400 bytes).
+int fmt_f6(double v, char* t)
 {
-    Out o(path);
-    if (!o.ok) return false;
-    for (long r = 0; r < rows; ++r)
-        for (long c = 0; c < cols; ++c) {
-            char sep = c == cols - 1 ? '\n' : '\t';
-            if constexpr (sizeof(V) == 4) o.i32((int32_t)a[r * cols + c], sep); else o.f64((double)a[r * cols + c], sep);
-        }
-    return o.close();
+    if (!(std::fabs(v) < 9.0e12)) return std::snprintf(t, 400, "%lf", v);
+    uint64_t bits;
+    std::memcpy(&bits, &v, sizeof(bits));
+    const bool neg = bits >> 63;
+    const int ex = (int)((bits >> 52) & 0x7ff);
+    uint64_t m = bits & ((1ull << 52) - 1);
+    int sh;   // |v| = m 2^-sh, sh >= 1 since |v| < 2^52
+    if (ex == 0) sh = 1074; else { m |= 1ull << 52; sh = 1075 - ex; }
+    uint64_t q = 0;   // round(|v| 1e6)
+    if (sh < 100) {
+        const unsigned __int128 num = (unsigned __int128)m * 1000000u;   // < 2^73
+        const unsigned __int128 one = (unsigned __int128)1;
+        const unsigned __int128 rem = num & ((one << sh) - 1), half = one << (sh - 1);
+        q = (uint64_t)(num >> sh);
+        if (rem > half || (rem == half && (q & 1))) ++q;
+    }
+    char d[32];
+    int n = 0, k = 0;
+    uint64_t ip = q / 1000000u;
+    uint32_t fp = (uint32_t)(q % 1000000u);
+    if (neg) t[n++] = '-';
+    do { d[k++] = (char)('0' + ip % 10); ip /= 10; } while (ip);
+    while (k) t[n++] = d[--k];
+    t[n++] = '.';
+    for (int j = 5; j >= 0; --j) { t[n + j] = (char)('0' + fp % 10); fp /= 10; }
+    return n + 6;
 }
 
-// matrix with a literal all-zero first row (cu:1705-1707, 1740-1742)
-bool write_zero_row_matrix(const std::string& path, const double* rest, long rows_rest, long cols)
+int fmt_i32(int32_t v, char* t)
 {
-    Out o(path);
-    if (!o.ok) return false;
-    for (long c = 0; c < cols; ++c) o.put(c == cols - 1 ? "0.000000\n" : "0.000000\t", 9);
-    for (long r = 0; r < rows_rest; ++r)
-        for (long c = 0; c < cols; ++c) o.f64(rest[r * cols + c], c == cols - 1 ? '\n' : '\t');
-    return o.close();
+    int n = 0, m = 0;
+    uint32_t u = v < 0 ? (uint32_t)(-(int64_t)v) : (uint32_t)v;
+    char d[12];
+    do { d[m++] = (char)('0' + u % 10); u /= 10; } while (u);
+    if (v < 0) t[n++] = '-';
+    while (m) t[n++] = d[--m];
+    return n;
+}
+
+// rows x cols matrix, tab-separated, newline per row (after an all-zero first
+// row, cu:1705-1707, 1740-1742, when zero_row).  Blocks of rows are formatted
+// by all threads and written in order.
+template <typename V>
+bool write_matrix(const std::string& path, const V* a, long rows, long cols, bool zero_row = false)
+{
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) { bq_set_error("cannot write " + path + ": " + std::strerror(errno)); return false; }
+    bool ok = true;
+    if (zero_row)
+        for (long c = 0; c < cols && ok; ++c) ok = std::fwrite(c == cols - 1 ? "0.000000\n" : "0.000000\t", 1, 9, f) == 9;
+    int T = 1;
+#ifdef _OPENMP
+    T = omp_get_max_threads();
+#endif
+    const long per_row = cols * (sizeof(V) == 4 ? 12 : 24) + 1;
+    const long rb = std::max(1L, (1L << 20) / per_row);   // rows per block, ~1 MB of text
+    std::vector<std::vector<char>> buf(T);
+    for (long r0 = 0; r0 < rows && ok; r0 += rb * T) {
+        #pragma omp parallel for num_threads(T) schedule(static, 1)
+        for (int t = 0; t < T; ++t) {
+            std::vector<char>& b = buf[t];
+            b.clear();
+            const long lo = r0 + t * rb, hi = std::min(rows, lo + rb);
+            char tmp[400];
+            for (long r = lo; r < hi; ++r)
+                for (long c = 0; c < cols; ++c) {
+                    int n;
+                    if constexpr (sizeof(V) == 4) n = fmt_i32((int32_t)a[r * cols + c], tmp);
+                    else n = fmt_f6((double)a[r * cols + c], tmp);
+                    tmp[n++] = c == cols - 1 ? '\n' : '\t';
+                    b.insert(b.end(), tmp, tmp + n);
+                }
+        }
+        for (int t = 0; t < T && ok; ++t)
+            if (!buf[t].empty()) ok = std::fwrite(buf[t].data(), 1, buf[t].size(), f) == buf[t].size();
+    }
+    if (std::fclose(f) != 0) ok = false;
+    if (!ok) bq_set_error("cannot write " + path + ": " + std::strerror(errno));
+    return ok;
+}
+
+// values on one line, each followed by sep (cu:1725-1727, 1838-1840)
+bool write_line(const std::string& path, const double* v, int n, char sep)
+{
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) { bq_set_error("cannot write " + path + ": " + std::strerror(errno)); return false; }
+    std::string line;
+    char t[400];
+    for (int j = 0; j < n; ++j) {
+        int m = fmt_f6(v[j], t);
+        t[m++] = sep;
+        line.append(t, (size_t)m);
+    }
+    bool ok = std::fwrite(line.data(), 1, line.size(), f) == line.size();
+    if (std::fclose(f) != 0) ok = false;
+    if (!ok) bq_set_error("cannot write " + path + ": " + std::strerror(errno));
+    return ok;
 }
 } // namespace
 
@@ -528,22 +567,14 @@ extern "C" busseq_status busseq_write_outputs(const busseq_result* r, const char
     ok = ok && write_matrix(p + "_post_gamma.txt", r->gamma, 2, B);
     ok = ok && write_matrix(p + "_post_alpha.txt", r->alpha, G, 1);
     ok = ok && write_matrix(p + "_post_L.txt", r->L + G, K - 1, G);
-    ok = ok && write_zero_row_matrix(p + "_post_beta.txt", r->beta + G, K - 1, G);
-    if (ok) {
-        Out o(p + "_post_p.txt");
-        ok = o.ok;
-        if (ok) { o.f64(r->ptau[0], '\t'); o.f64(r->ptau[1], '\t'); ok = o.close(); }
-    }
-    ok = ok && write_zero_row_matrix(p + "_post_nu.txt", r->nu + G, B - 1, G);
+    ok = ok && write_matrix(p + "_post_beta.txt", r->beta + G, K - 1, G, true);
+    ok = ok && write_line(p + "_post_p.txt", r->ptau, 2, '\t');
+    ok = ok && write_matrix(p + "_post_nu.txt", r->nu + G, B - 1, G, true);
     ok = ok && write_matrix(p + "_post_delta.txt", r->delta, N, 1);
     ok = ok && write_matrix(p + "_post_phi.txt", r->phi, B, G);
     ok = ok && write_matrix(p + "_post_W.txt", r->W, N, 1);
     ok = ok && write_matrix(p + "_post_pi.txt", r->pi, B, K);
-    if (ok) {
-        Out o(p + "_BIC.txt");
-        ok = o.ok;
-        if (ok) { o.f64(r->bic, '\n'); ok = o.close(); }
-    }
+    ok = ok && write_line(p + "_BIC.txt", &r->bic, 1, '\n');
     if (ok && print_preserved) {
         if (!r->tr_alpha) { bq_set_error("write_outputs: traces were not kept (keep_traces = 0)"); return BUSSEQ_ERR_ARG; }
         ok = ok && write_matrix(p + "_preserved_gamma.txt", r->tr_gamma, 2 * B, P);

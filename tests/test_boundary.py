@@ -285,3 +285,37 @@ def test_cli_count_cache(tmp_path):
     # a binary count file is accepted as -c directly
     p = cli(*args[:11], str(cache), *args[12:-1])
     assert "Start initialization." in p.stdout
+
+
+def test_f64_writer_equals_printf_rounding(tmp_path):
+    # the writers' own "%lf" formatting against Python's correctly rounded "%f"
+    # (round half to even on the exact binary value, as glibc's printf)
+    rng = np.random.default_rng(7)
+    mags = 10.0 ** rng.uniform(-12, 12.9, size=150000)
+    vals = np.concatenate([
+        mags * rng.choice([-1.0, 1.0], size=mags.size),
+        np.arange(-4096, 4096) / 128.0, np.arange(-4096, 4096) / 1024.0,     # exact ties at the 6th decimal
+        (np.arange(1, 2000) * 15625.0 + 0.5) / 1e6,                           # near ties
+        [0.0, -0.0, 5e-324, -5e-324, 2.2250738585072014e-308, 0.0000005, -0.0000005, 0.0000015,
+         8.999999999999e12, -8.999999999999e12, 9.0e12, 1.5e13, -3.7e19, 1e300, np.inf, -np.inf, np.nan,
+         0.1, 0.7, 1.0000005, 123456.7890125, 2.5e-7, 7.5e-7]])
+    N = vals.size
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_int32 if dt == np.int32 else C.c_double))
+
+    r = _Result()
+    r.B, r.N, r.G, r.K, r.n_pres = 2, N, 1, 2, 0
+    r.X = arr(np.zeros(N), np.int32); r.Z = arr(np.zeros(N), np.int32); r.W = arr(np.zeros(N), np.int32)
+    r.delta = arr(vals, np.float64)
+    for k, n in (("gamma", 4), ("alpha", 1), ("L", 2), ("beta", 2), ("ptau", 2), ("nu", 2), ("phi", 2), ("pi", 4)):
+        setattr(r, k, arr(np.zeros(n), np.float64))
+    prefix = str(tmp_path / "o")
+    assert _load().busseq_write_outputs(C.byref(r), prefix.encode(), 0) == 0
+    got = open(prefix + "_post_delta.txt").read().split("\n")[:-1]
+    want = ["%f" % v for v in vals]
+    bad = [(v, g, w) for v, g, w in zip(vals, got, want) if g != w]
+    assert not bad, bad[:5]
