@@ -45,7 +45,7 @@
     Q double *alpha, *beta, *nu, *delta, *phi, *gamma, *pi, *ptau; \
     Q double *mu_alpha, *mu_nu, *mu_delta; \
     /* precision-T copies of the parameters */ \
-    Q void *ea, *eb, *bt, *en, *ed, *ph;  /* bt: [K*G][2] (beta, expm1 beta) */ \
+    Q void *ea, *eb, *bt, *en, *ed, *ph;  /* bt: [G][bt_stride(K)][2] (beta_k, expm1 beta_k), k = 1..K-1 */ \
     /* proposals and their precision-T constants */ \
     Q double *alpha_p, *beta_p, *nu_p, *phi_p, *gamma_p; \
     Q void *da, *ca, *db, *cb, *dn, *cn; \

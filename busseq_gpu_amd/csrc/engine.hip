@@ -66,6 +66,12 @@ __device__ __forceinline__ float ttphi(int xi, float x, float ph, float pp, floa
 __device__ __forceinline__ double ttphi(int xi, double x, double ph, double pp, double mu, double d, double dop, double ip, double lr, double dl)
 { return bq_tphi_f64(xi, x, ph, pp, mu, d, dop, ip, lr, dl); }
 
+// (beta_kg, expm1 beta_kg) pairs of the subtype draw, gene-major: gene g holds
+// the pairs of k = 1..K-1 at pair index g * bt_stride(K) + k - 1; the stride is
+// K - 1 rounded up to even, so a gene's pairs start 16-byte aligned in single
+// precision and are read with one lane offset and immediate offsets.
+__host__ __device__ __forceinline__ int bt_stride(int K) { return K & ~1; }
+
 // exact wave-wide sum of a fixed-point value (wrapping uint64 adds are associative)
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 {
@@ -192,16 +198,18 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
     typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wave's cell, made wave-uniform so that every row and parameter base
+    // below is a scalar and each load is base + 32-bit lane offset
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (i >= c.N) return;
     const int t = *c.iter;
     const int G = c.G, b = c.batch[i];
-    const T* ea = (const T*)c.ea;
-    const T* en = (const T*)c.en + (size_t)b * G;
-    const T* ph = (const T*)c.ph + (size_t)b * G;
-    const T2* be = (const T2*)c.bt;          // (beta_kg, expm1(beta_kg)) pairs
+    const BQ_GLOBAL char* ea = (const BQ_GLOBAL char*)c.ea;
+    const BQ_GLOBAL char* en = (const BQ_GLOBAL char*)c.en + (size_t)b * G * sizeof(T);
+    const BQ_GLOBAL char* ph = (const BQ_GLOBAL char*)c.ph + (size_t)b * G * sizeof(T);
+    const BQ_GLOBAL char* be = (const BQ_GLOBAL char*)c.bt;   // (beta_kg, expm1(beta_kg)) pairs
     const T edi = ((const T*)c.ed)[i];
-    const int32_t* row = (const int32_t*)(c.S + (size_t)i * c.Gp);
+    const BQ_GLOBAL char* row = (const BQ_GLOBAL char*)(c.S + (size_t)i * c.Gp);
 
     uint64_t acc[KT];
 #pragma unroll
@@ -212,14 +220,16 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
 #pragma unroll
         for (int k = 0; k < KT; ++k) part[k] = 0.0;
         for (unsigned g = (unsigned)(c0 + lane); g < c1; g += 64) {
-            int32_t s = row[g];
+            const unsigned o = g * (unsigned)sizeof(T);
+            int32_t s = *(const BQ_GLOBAL int32_t*)(row + g * 4u);
             int xi = s < 0 ? ~s : s;
             T x = (T)xi;
-            T phi = ph[g];
+            T phi = *(const BQ_GLOBAL T*)(ph + o);
             T2 v[KT];
+            const BQ_GLOBAL T* bg = (const BQ_GLOBAL T*)(be + g * (unsigned)(bt_stride(KT) * sizeof(T2)));
 #pragma unroll
-            for (int k = 1; k < KT; ++k) v[k] = be[(unsigned)k * (unsigned)G + g];
-            T mu0 = tmu(ea[g], (T)1, en[g], edi);
+            for (int k = 1; k < KT; ++k) { v[k].x = bg[2 * k - 2]; v[k].y = bg[2 * k - 1]; }
+            T mu0 = tmu(*(const BQ_GLOBAL T*)(ea + o), (T)1, *(const BQ_GLOBAL T*)(en + o), edi);
             T r0 = tratio(mu0, phi);
 #pragma unroll
             for (int k = 1; k < KT; ++k) part[k] += (double)ttw(x, phi, r0, v[k].x, v[k].y);
@@ -1026,8 +1036,9 @@ __global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
     if (acc) {
         c.beta[q] = bp;
         ((T*)c.eb)[q] = (T)bq_exp(bp);
-        ((T*)c.bt)[2 * q] = (T)bp;
-        ((T*)c.bt)[2 * q + 1] = (T)bq_expm1(bp);
+        const size_t pq = (size_t)g * bt_stride(K) + (k - 1);
+        ((T*)c.bt)[2 * pq] = (T)bp;
+        ((T*)c.bt)[2 * pq + 1] = (T)bq_expm1(bp);
     }
 }
 
@@ -1413,11 +1424,18 @@ double f_exp(double x) { return bq_exp(x); }
 
 // (beta, expm1(beta)) pairs in precision T, the layout k_w reads
 template <typename T>
-void to_pairs(std::vector<char>& out, const std::vector<double>& v)
+void to_pairs(std::vector<char>& out, const std::vector<double>& beta, int K, int G)
 {
-    out.resize(2 * v.size() * sizeof(T));
+    const int st = bt_stride(K);
+    out.assign(2 * (size_t)G * st * sizeof(T), 0);
     T* p = (T*)out.data();
-    for (size_t j = 0; j < v.size(); ++j) { p[2 * j] = (T)v[j]; p[2 * j + 1] = (T)bq_expm1(v[j]); }
+    for (int g = 0; g < G; ++g)
+        for (int k = 1; k < K; ++k) {
+            const double v = beta[(size_t)k * G + g];
+            const size_t q = (size_t)g * st + (k - 1);
+            p[2 * q] = (T)v;
+            p[2 * q + 1] = (T)bq_expm1(v);
+        }
 }
 
 // precision-T copies of a chain's parameters, computed on the host with the
@@ -1432,14 +1450,14 @@ busseq_status upload_tcopies(busseq_sampler* s, int ch, const std::vector<double
     if (s->prec == 32) {
         to_t<float>(buf, alpha, f_exp); HIPCHK(up(c.ea));
         to_t<float>(buf, beta, f_exp); HIPCHK(up(c.eb));
-        to_pairs<float>(buf, beta); HIPCHK(up(c.bt));
+        to_pairs<float>(buf, beta, c.K, c.G); HIPCHK(up(c.bt));
         to_t<float>(buf, nu, f_exp); HIPCHK(up(c.en));
         to_t<float>(buf, delta, f_exp); HIPCHK(up(c.ed));
         to_t<float>(buf, phi, nullptr); HIPCHK(up(c.ph));
     } else {
         to_t<double>(buf, alpha, f_exp); HIPCHK(up(c.ea));
         to_t<double>(buf, beta, f_exp); HIPCHK(up(c.eb));
-        to_pairs<double>(buf, beta); HIPCHK(up(c.bt));
+        to_pairs<double>(buf, beta, c.K, c.G); HIPCHK(up(c.bt));
         to_t<double>(buf, nu, f_exp); HIPCHK(up(c.en));
         to_t<double>(buf, delta, f_exp); HIPCHK(up(c.ed));
         to_t<double>(buf, phi, nullptr); HIPCHK(up(c.ph));
@@ -1667,7 +1685,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.pi = s->dalloc<double>((size_t)B * K); c.ptau = s->dalloc<double>(2);
             c.mu_alpha = s->dalloc<double>(G); c.mu_nu = s->dalloc<double>(B); c.mu_delta = s->dalloc<double>(N);
             c.ea = s->dalloc<char>(ts * G); c.eb = s->dalloc<char>(ts * K * G);
-            c.bt = s->dalloc<char>(2 * ts * K * G); c.en = s->dalloc<char>(ts * B * G); c.ed = s->dalloc<char>(ts * N);
+            c.bt = s->dalloc<char>(2 * ts * (size_t)bt_stride(K) * G); c.en = s->dalloc<char>(ts * B * G); c.ed = s->dalloc<char>(ts * N);
             c.ph = s->dalloc<char>(ts * B * G);
             c.alpha_p = s->dalloc<double>(G); c.beta_p = s->dalloc<double>((size_t)K * G);
             c.nu_p = s->dalloc<double>((size_t)B * G); c.phi_p = s->dalloc<double>((size_t)B * G);
@@ -1944,15 +1962,19 @@ extern "C" busseq_status busseq_copy_probe(size_t bytes, int device, double* ms)
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    // one warm-up launch, then the average of ten
+    hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, nullptr, (const uint4*)a, (uint4*)b, n);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipEventRecord(e0, nullptr));
-    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, nullptr, (const uint4*)a, (uint4*)b, n);
+    for (int r = 0; r < 10; ++r)
+        hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, nullptr, (const uint4*)a, (uint4*)b, n);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, nullptr));
     HIPCHK(hipEventSynchronize(e1));
     float f = 0.f;
     HIPCHK(hipEventElapsedTime(&f, e0, e1));
-    *ms = f;
+    *ms = f / 10.0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIPCHK(hipFree(a));

@@ -159,7 +159,8 @@ busseq_status busseq_autotune_batch(const busseq_problem* problems, int n_proble
 int busseq_kernel_count(void);
 const char* busseq_kernel_name(int j);
 busseq_status busseq_kernel_times(busseq_sampler* s, int n, double* ms);
-/* one streaming copy of `bytes` bytes on the device (profiler counter calibration) */
+/* streaming copies of `bytes` bytes on the device: one warm-up launch, then *ms = the
+ * average of ten launches (profiler counter calibration, achievable bandwidth) */
 busseq_status busseq_copy_probe(size_t bytes, int device, double* ms);
 
 /* message of the last failing call on this thread */
