@@ -66,6 +66,14 @@ __device__ __forceinline__ float ttphi(int xi, float x, float ph, float pp, floa
 __device__ __forceinline__ double ttphi(int xi, double x, double ph, double pp, double mu, double d, double dop, double ip, double lr, double dl)
 { return bq_tphi_f64(xi, x, ph, pp, mu, d, dop, ip, lr, dl); }
 
+// value of type V at byte offset o (32-bit) from a wave-uniform base: the form
+// the compiler emits as a global load with a scalar base and a lane offset
+template <typename V>
+__device__ __forceinline__ V ld_at(const BQ_GLOBAL void* base, unsigned o)
+{
+    return *(const BQ_GLOBAL V*)((const BQ_GLOBAL char*)base + o);
+}
+
 // (beta_kg, expm1 beta_kg) pairs of the subtype draw, gene-major: gene g holds
 // the pairs of k = 1..K-1 at pair index g * bt_stride(K) + k - 1; the stride is
 // K - 1 rounded up to even, so a gene's pairs start 16-byte aligned in single
@@ -433,7 +441,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ XzItem queue[4][XZ_QCAP];
     __shared__ double s_odds[XZ_SPT];
     __shared__ double s_inv[BQ_INV_N];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    // wave and cell are wave-uniform: everything derived from them stays in scalar registers
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = blockIdx.x * 4 + wv;
     // the grid covers the largest chain of a batch: blocks past this chain's cells leave
     if (blockIdx.x * 4 >= c.N) return;
@@ -1088,17 +1098,18 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // wave-uniform cell: row and parameter bases are scalars, loads take 32-bit lane offsets
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (i >= c.N) return;
     const int b = c.batch[i];
     if (i == c.boff[b]) return;
     const int t = *c.iter;
     const int G = c.G, k = c.W[i];
-    const T* ea = (const T*)c.ea;
-    const T* eb = (const T*)c.eb + (size_t)k * G;
-    const T* en = (const T*)c.en + (size_t)b * G;
-    const T* ph = (const T*)c.ph + (size_t)b * G;
-    T* ed = (T*)c.ed;
+    const BQ_GLOBAL T* ea = (const BQ_GLOBAL T*)c.ea;
+    const BQ_GLOBAL T* eb = (const BQ_GLOBAL T*)c.eb + (size_t)k * G;
+    const BQ_GLOBAL T* en = (const BQ_GLOBAL T*)c.en + (size_t)b * G;
+    const BQ_GLOBAL T* ph = (const BQ_GLOBAL T*)c.ph + (size_t)b * G;
+    BQ_GLOBAL T* ed = (BQ_GLOBAL T*)c.ed;
     const T edi = ed[i];
     const double del = c.delta[i];
     bq_rng r;
@@ -1112,13 +1123,16 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
         const int c1 = c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G;
         double part = 0.0;
         // four genes of the lane class per step, loaded together, added in gene order
-        for (int g0 = c0 + lane; g0 < c1; g0 += 256) {
+        for (unsigned g0 = (unsigned)(c0 + lane); g0 < (unsigned)c1; g0 += 256) {
             int32_t sv[4];
             T pv[4], av[4], bv[4], nv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int g = g0 + 64 * u;
-                if (g < c1) { sv[u] = row[g]; pv[u] = ph[g]; av[u] = ea[g]; bv[u] = eb[g]; nv[u] = en[g]; }
+                const unsigned g = g0 + 64u * u, o = g * (unsigned)sizeof(T);
+                if (g < (unsigned)c1) {
+                    sv[u] = ld_at<int32_t>(row, g * 4u);
+                    pv[u] = ld_at<T>(ph, o); av[u] = ld_at<T>(ea, o); bv[u] = ld_at<T>(eb, o); nv[u] = ld_at<T>(en, o);
+                }
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
