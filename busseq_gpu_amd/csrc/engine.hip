@@ -1,8 +1,8 @@
 // engine.hip -- gfx950 kernels of one BUSseq Gibbs sweep and the host driver
 // that allocates the chains, captures the sweep into a hipGraph and replays it.
 //
-// Sweep (one hipGraph, sixteen launches, chains on blockIdx.z):
-//   k_prep       alpha/gamma proposals, zero the per-sweep counters
+// Sweep (one hipGraph, fifteen launches, chains on blockIdx.z):
+//   k_prep       alpha/gamma proposals, X/Z tables, zero the per-sweep counters
 //   k_w          W_i: categorical full conditional, one wave per cell     [row pass]
 //   k_plan       stable sort of cells by (b, W), tiles
 //   k_xz         X/Z imputation of the Y = 0 entries, gamma histogram     [row pass]
@@ -16,8 +16,11 @@
 //   k_delta      delta proposal, row sum and accept, one wave per cell   [row pass]
 //   k_phi        phi partial sums with the new delta                     [tile pass]
 //   k_phi_dec    phi accept
-//   k_record     posterior accumulators and traces after burn-in
-//   k_tick       iteration counter
+//   k_record     posterior accumulators and traces after burn-in; subtype
+//                counts cleared; iteration counter (by the last block)
+// Three kernels have no data dependence on the launch before them and run on
+// a second branch of the graph: k_prep beside k_w, k_plan beside k_xz and
+// k_xz_tail, k_gamma_ptau beside k_beta_nu.
 //
 // Row passes: genes are cut into chunks of BQ_CHUNK = 1024; lane l of the wave
 // owning cell i sums the genes g = l (mod 64) of each chunk in double
@@ -186,7 +189,6 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
         const int b = j / H, x = j - b * H;
         c.odds_tab[j] = bq_xz_odds(c.gamma[b], c.gamma[c.B + b], (double)x);
     }
-    if (j < c.B * c.K * BQ_STRIPES) c.cnt[j] = 0;
     if (j < 2) c.lstat[j] = 0;
     if (j < c.K * c.B * c.G) {
         // ((e^alpha e^beta) e^nu): the first two products of the mean, in its order
@@ -1219,14 +1221,9 @@ __global__ __launch_bounds__(256) void k_phi_dec(const ChainDev* __restrict__ cd
 // ---------------------------------------------------------------------------
 // k_record: posterior accumulators and traces (cu:892-936, 1547-1554)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_record(const ChainDev* __restrict__ cds)
+__device__ __forceinline__ void record_sweep(const ChainG& c, int t, int j, int G, int K, int B, int N, int P)
 {
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    const int t = *c.iter;
-    if (t < c.n_burnin) return;
     const int r = t - c.n_burnin;
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    const int G = c.G, K = c.K, B = c.B, N = c.N, P = c.n_pres;
     const bool tr = c.traces && r < P;
     if (j == 0) *c.n_rec += 1;
     if (j < 2 * B) {
@@ -1271,9 +1268,31 @@ __global__ __launch_bounds__(256) void k_record(const ChainDev* __restrict__ cds
     }
 }
 
-__global__ void k_tick(const ChainDev* __restrict__ cds)
+
+// The subtype counts of this sweep (read by k_plan and the pi draw) are cleared
+// here for the next one.  Every block then counts itself done; the last block
+// of a chain, after all the others have read the iteration index, advances it.
+__device__ __forceinline__ void record_done(const ChainG& c)
 {
-    if (threadIdx.x == 0) *cds[blockIdx.z].iter += 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (gatomic_add(c.rec_done, 1) == (int)gridDim.x - 1) {
+            *c.rec_done = 0;
+            *c.iter += 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_record(const ChainDev* __restrict__ cds)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    const int t = *c.iter;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int G = c.G, K = c.K, B = c.B, N = c.N, P = c.n_pres;
+    if (j < B * K * BQ_STRIPES) c.cnt[j] = 0;
+    if (t >= c.n_burnin) record_sweep(c, t, j, G, K, B, N, P);
+    record_done(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1341,6 +1360,8 @@ struct busseq_sampler {
     ChainDev* d_cd = nullptr;
     std::vector<std::pair<int, int>> krange;   // runs of consecutive chains with equal K
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;    // second branch of the captured sweep (default priority)
+    hipEvent_t fev[6] = {};        // fork and join points of the captured sweep
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<void*> allocs;
@@ -1363,8 +1384,8 @@ namespace {
 
 static const char* const kKernelNames[] = {
     "k_prep", "k_w", "k_plan", "k_xz", "k_xz_tail", "k_alpha", "k_alpha_dec", "k_gamma_ptau",
-    "k_beta_nu", "k_b_dec", "k_n_dec", "k_delta", "k_phi", "k_phi_dec", "k_record", "k_tick"};
-static const int kNumKernels = 16;
+    "k_beta_nu", "k_b_dec", "k_n_dec", "k_delta", "k_phi", "k_phi_dec", "k_record"};
+static const int kNumKernels = 15;
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
@@ -1386,28 +1407,46 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     const dim3 g_rec(nblk(s->maxRec), 1, C);
     int q = 0;
 #define EV() do { if (ev) HIPCHK(hipEventRecord(ev[q], st)); ++q; } while (0)
-    EV(); hipLaunchKernelGGL(k_prep<T>, g_prep, blk, 0, st, cd);
+    // Second branch of the captured sweep.  With ev (per-kernel timing) every
+    // kernel runs in order on one stream, so that the events bracket each.
+    const bool fork = ev == nullptr && s->side != nullptr;
+    hipStream_t sd = fork ? s->side : st;
+#define FORK(e) do { if (fork) { HIPCHK(hipEventRecord(s->fev[e], st)); HIPCHK(hipStreamWaitEvent(sd, s->fev[e], 0)); } } while (0)
+#define JOIN(e) do { if (fork) { HIPCHK(hipEventRecord(s->fev[e], sd)); HIPCHK(hipStreamWaitEvent(st, s->fev[e], 0)); } } while (0)
+    // k_prep beside k_w: it reads none of W and the subtype counts, the only
+    // things k_w writes, and k_w reads none of its outputs
+    FORK(0);
+    EV(); hipLaunchKernelGGL(k_prep<T>, g_prep, blk, 0, sd, cd);
     EV();
     for (const auto& kr : s->krange) {
         int maxn = 0;
         for (int c = kr.first; c < kr.first + kr.second; ++c) maxn = s->ch[c].N > maxn ? s->ch[c].N : maxn;
         launch_w<T>(s->ch[kr.first].K, dim3((maxn + 3) / 4, 1, kr.second), blk, st, cd + kr.first);
     }
-    EV(); hipLaunchKernelGGL(k_plan, g_plan, dim3(PLAN_THREADS), 0, st, cd);
+    JOIN(1);   // k_xz reads k_prep's tables and cleared counters
+    // k_plan (tiles and order of the cells, from W) beside the X/Z kernels,
+    // which read neither
+    FORK(2);
+    EV(); hipLaunchKernelGGL(k_plan, g_plan, dim3(PLAN_THREADS), 0, sd, cd);
     EV(); hipLaunchKernelGGL(k_xz<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_xz_tail<T>, dim3((s->maxN + 4 * TAIL_CPW - 1) / (4 * TAIL_CPW), 1, C), blk, 0, st, cd);
+    JOIN(3);   // k_alpha reads the tiles
     EV(); hipLaunchKernelGGL(k_alpha<T>, g_tile, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_adec, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, st, cd);
+    // k_gamma_ptau (gamma, p, tau0, pi) beside k_beta_nu, which reads none of them
+    FORK(4);
+    EV(); hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, sd, cd);
     EV(); hipLaunchKernelGGL(k_beta_nu<T>, g_tile, blk, 0, st, cd);
+    JOIN(5);   // k_b_dec reads tau0
     EV(); hipLaunchKernelGGL(k_b_dec<T>, g_dec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_n_dec<T>, g_dec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_phi_dec<T>, g_dec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_record, g_rec, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_tick, dim3(1, 1, C), dim3(64), 0, st, cd);
     EV();
+#undef FORK
+#undef JOIN
 #undef EV
     HIPCHK(hipGetLastError());
     return BUSSEQ_OK;
@@ -1650,10 +1689,16 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
     s->TR = c0->tile_rows > 0 ? c0->tile_rows : 64;
     s->dev = c0->device;
     if (hipSetDevice(s->dev) != hipSuccess) { bq_set_error("hipSetDevice failed (no GPU?)"); return fail(BUSSEQ_ERR_DEVICE); }
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) != hipSuccess) {
         bq_set_error("hipStreamCreate failed");
         return fail(BUSSEQ_ERR_DEVICE);
     }
+    for (hipEvent_t& e : s->fev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            bq_set_error("hipEventCreate failed");
+            return fail(BUSSEQ_ERR_DEVICE);
+        }
     const size_t ts = s->prec == 32 ? 4 : 8;
     const int TR = s->TR;
     for (int p = 0; p < n_probs; ++p) {
@@ -1725,6 +1770,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.odds_tab = s->dalloc<double>((size_t)B * H);
             c.beta_acc = s->dalloc<int32_t>((size_t)K * G);
             c.n_rec = s->dalloc<int>(1);
+            c.rec_done = s->dalloc<int>(1);
             c.s_gamma = s->dalloc<double>(2 * (size_t)B); c.s_alpha = s->dalloc<double>(G);
             c.s_beta = s->dalloc<double>((size_t)K * G); c.s_nu = s->dalloc<double>((size_t)B * G);
             c.s_delta = s->dalloc<double>(N); c.s_phi = s->dalloc<double>((size_t)B * G);
@@ -1775,7 +1821,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             mx(s->maxAdec, nA + nblk((long)G * ((K - 1) + (B - 1) + B)));
             mx(s->maxDec, nA * (K - 1)); mx(s->maxDec, nA * (B - 1)); mx(s->maxDec, nA * B);
             mx(s->maxRec, N); mx(s->maxRec, (long)K * G); mx(s->maxRec, (long)B * G); mx(s->maxRec, (long)B * K);
-            mx(s->maxRec, 2L * B);
+            mx(s->maxRec, 2L * B); mx(s->maxRec, (long)B * K * BQ_STRIPES);
         }
     }
     s->C = (int)s->hd.size();
@@ -1810,6 +1856,9 @@ extern "C" void busseq_destroy(busseq_sampler* s)
     if (s->graph) (void)hipGraphDestroy(s->graph);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->d_cd) (void)hipFree(s->d_cd);
+    for (hipEvent_t e : s->fev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->side) (void)hipStreamDestroy(s->side);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
