@@ -83,6 +83,15 @@ __device__ __forceinline__ V ld_at(const BQ_GLOBAL void* base, unsigned o)
 // precision and are read with one lane offset and immediate offsets.
 __host__ __device__ __forceinline__ int bt_stride(int K) { return K & ~1; }
 
+// value of lane k (a constant or wave-uniform index) in every lane
+__device__ __forceinline__ double read_lane(double v, int k)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), k);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // exact wave-wide sum of a fixed-point value (wrapping uint64 adds are associative)
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 {
@@ -247,18 +256,30 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
 #pragma unroll
         for (int k = 1; k < KT; ++k) acc[k] += bq_fix(part[k]);
     }
-    double lp[KT];
+    // Subtype k's logit, log pi and exponential are evaluated once, in lane k,
+    // instead of in every lane; the maximum, the total and the search then run
+    // in subtype order on the values read back from the lanes, as a serial loop.
+    double lpl = -BQ_INF;
+    {
+        uint64_t mine = 0;
+#pragma unroll
+        for (int k = 1; k < KT; ++k) {
+            const uint64_t a = wave_sum_u64(acc[k]);
+            if (lane == k) mine = a;
+        }
+        if (lane < KT) lpl = bq_unfix(mine) + bq_log(c.pi[b * KT + lane]);
+    }
     double mx = -BQ_INF;
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
-        uint64_t a = k > 0 ? wave_sum_u64(acc[k]) : 0;
-        double lk = bq_unfix(a);
-        lp[k] = lk + bq_log(c.pi[b * KT + k]);
-        if (lp[k] > mx) mx = lp[k];
+        const double v = read_lane(lpl, k);
+        if (v > mx) mx = v;
     }
+    const double el = bq_exp(lpl - mx);
+    double lp[KT];
     double tot = 0.0;
 #pragma unroll
-    for (int k = 0; k < KT; ++k) { lp[k] = bq_exp(lp[k] - mx); tot += lp[k]; }
+    for (int k = 0; k < KT; ++k) { lp[k] = read_lane(el, k); tot += lp[k]; }
     double tgt = ph_uniform(c, (uint32_t)i, 0u, t, BQ_PH_W) * tot, cum = 0.0;
     int kk = KT - 1, found = 0;
 #pragma unroll
