@@ -78,6 +78,14 @@ def exp_e(x):
     return (mp.expm1(x) - x) / x ** 2
 
 
+def stirling_q(u):
+    # Q(u) = w S(w) with u = 1/w^2 and S(w) = lgamma(w) - (w - 1/2) log w + w - log(2 pi)/2
+    if u == 0:
+        return mp.mpf(1) / 12
+    w = 1 / mp.sqrt(u)
+    return w * (mp.loggamma(w) - (w - mp.mpf(1) / 2) * mp.log(w) + w - mp.log(2 * mp.pi) / 2)
+
+
 def main():
     import numpy as np
     f32 = lambda v: float(np.float32(v))
@@ -100,8 +108,14 @@ def main():
     for name, cast, deg in (("EXP_E_F64", f64, 11),):
         co, w = report(name, exp_e, -r, r, deg, cast, rel_exp)
         out.append((name, co, w))
+    # Stirling remainder for w >= 1 (u in (0, 1]); the error is absolute, in S = Q(u)/w
+    def abs_s(u, q):
+        return (q - stirling_q(u)) * mp.sqrt(u)
+
+    co, w = report("STIRLING_Q_F32", stirling_q, 0, 1, 6, f32, abs_s)
+    out.append(("STIRLING_Q_F32", co, w))
     for name, co, w in out:
-        print(f"/* {name}: degree {len(co)-1}, max rel err of the rounded polynomial {mp.nstr(w, 3)} */")
+        print(f"/* {name}: degree {len(co)-1}, max err of the rounded polynomial {mp.nstr(w, 3)} */")
         print(f"#define {name} \\")
         print("    " + ", \\\n    ".join(mp.nstr(v, 21, min_fixed=0, max_fixed=0) for v in co))
         print()

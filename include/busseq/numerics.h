@@ -381,32 +381,39 @@ BQ_FN float bq_log1pf(float y)
     return res;
 }
 
-/* Stirling remainder for w >= 4 in single precision (6 terms, abs err < 5e-10) */
+/* Stirling remainder S(w) = lgamma(w) - (w - 1/2) log w + w - log(2 pi)/2 for
+ * w >= 1 in single precision, from iw = 1/w: iw Q(iw^2), Q the degree-6
+ * Chebyshev interpolant of w S(w) in u = 1/w^2 on [0, 1]
+ * (busseq_gpu_amd/csrc/gen_coeffs.py; abs err of S < 6.1e-9 with the rounded
+ * coefficients, < 1e-8 evaluated in single precision).  Valid down to w = 1,
+ * so the arguments of the phi term, phi + x with x >= 1, are never shifted. */
 BQ_FN float bq_stirling_s32(float iw)
 {
     float z = iw * iw;
-    float p = -0.0019175269408151507f;
-    p = __builtin_fmaf(p, z, 0.0008417508215643466f);
-    p = __builtin_fmaf(p, z, -0.0005952381179668009f);
-    p = __builtin_fmaf(p, z, 0.0007936508045531809f);
-    p = __builtin_fmaf(p, z, -0.0027777778450399637f);
-    p = __builtin_fmaf(p, z, 0.0833333358168602f);
+    float p = 3.96799550799187272787e-5f;
+    p = __builtin_fmaf(p, z, -1.71036081155762076378e-4f);
+    p = __builtin_fmaf(p, z, 3.40970960678532719612e-4f);
+    p = __builtin_fmaf(p, z, -4.83421026729047298431e-4f);
+    p = __builtin_fmaf(p, z, 7.78905232436954975128e-4f);
+    p = __builtin_fmaf(p, z, -2.77695455588400363922e-3f);
+    p = __builtin_fmaf(p, z, 8.33333283662796020508e-2f);
     return p * iw;
 }
 
 /*
  * lgamma(z + d) - lgamma(z) for z > 0 and z + d > 0, computed without the
  * cancellation of two large lgamma values:
- *   both arguments are shifted up to >= Z0 with the recurrence, then
- *   d*log(z+d) + (z-1/2)*log1p(d/z) - d + S(z+d) - S(z)   (Stirling).
+ *   d*log(z+d) + (z-1/2)*log1p(d/z) - d + S(z+d) - S(z)   (Stirling),
+ * with both arguments first shifted up to >= 1 by the recurrence where
+ * needed (never for the phi term: there z, z + d > 1).
  */
 BQ_FN float bq_lgamma_diff32(float z, float d)
 {
     float zp = z + d;
     float pz = 1.0f, pzp = 1.0f;
     int shifted = 0;
-    while (z < 4.0f || zp < 4.0f) { pz *= z; pzp *= zp; z += 1.0f; zp += 1.0f; shifted = 1; }
-    /* both reciprocals from one division (z, zp >= 4: no overflow; ~2 ulp each) */
+    while (z < 1.0f || zp < 1.0f) { pz *= z; pzp *= zp; z += 1.0f; zp += 1.0f; shifted = 1; }
+    /* both reciprocals from one division (z, zp >= 1; ~2 ulp each) */
     float w = 1.0f / (z * zp);
     float iz = zp * w, izp = z * w;
     float core = __builtin_fmaf(d, bq_logf_norm(zp), __builtin_fmaf(z - 0.5f, bq_log1pf(d * iz), -d));

@@ -92,6 +92,26 @@ def test_lgamma_diff_vs_mpmath(gold):
     assert (e32 / np.maximum(np.abs(ref), 1.0)).max() < 2e-6
 
 
+def test_lgamma_diff32_on_the_phi_term_domain():
+    # the phi term evaluates lgamma(phi' + x) - lgamma(phi + x) with x >= 1 and
+    # phi' ~ Gamma(phi, 1): arguments above 1, where the fitted Stirling remainder
+    # is used without a shift; checked against 40-digit mpmath at the float arguments
+    import mpmath as mp
+    rng = np.random.default_rng(11)
+    n = 1500
+    phi = np.exp(rng.uniform(np.log(0.02), np.log(60), n)).astype(np.float32)
+    x = np.where(rng.random(n) < 0.5, rng.integers(1, 2001, n), rng.integers(1, 12, n)).astype(np.float32)
+    phip = rng.gamma(phi.astype(np.float64), 1.0).astype(np.float32)
+    z = (phi + x).astype(np.float32)
+    d = (phip - phi).astype(np.float32)
+    assert z.min() > 1.0
+    got = po.math("lgamma_diff32", z.astype(np.float64), d.astype(np.float64))
+    with mp.workdps(40):
+        ref = np.array([float(mp.loggamma(mp.mpf(float(a)) + mp.mpf(float(b))) - mp.loggamma(mp.mpf(float(a))))
+                        for a, b in zip(z, d)])
+    assert (np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)).max() < 5e-7
+
+
 def test_single_precision_logs_vs_mpmath(gold):
     x = gold["log_x"].astype(np.float32)
     x = x[np.isfinite(x) & (x > 0)]
