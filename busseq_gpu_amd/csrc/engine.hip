@@ -422,14 +422,15 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_plan(const ChainDev* __restric
 // The update of every entry is bq_xz_update; an entry is written back only
 // when its state changes.
 // ---------------------------------------------------------------------------
+template <typename T>
 struct XzItem {
-    int p;                 // index of the entry in the zero-entry list
-    int g;
+    uint32_t pg;           // index of the entry in its cell's zero-entry list (low 16 bits), gene << 16
     int32_t s;
     uint32_t b0, b1, b2;   // unread words of the entry's Philox block after the Z draw
-    double mu;
+    T mu;                  // the proposal mean as computed, in precision T
     double phi;
 };
+static_assert(sizeof(XzItem<float>) == 32, "32-byte queue entries in single precision");
 
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -455,13 +456,13 @@ static_assert(BQ_HLO == 64, "k_xz keeps one low histogram bin per lane");
 #define XZ_QCAP 192  // queue entries per wave: two stacks of < 128 entries, fewer than 192 together
 #define XZ_MU_SPLIT 6.0  // proposals with a larger mean go to the high stack
 
-// four waves per SIMD: the register budget of 128 holds without spills
+// five waves per SIMD: 32-byte queue entries keep the block's LDS under a fifth of the CU's
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xz(const ChainDev* __restrict__ cds)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_xz(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ uint32_t hist[4][BQ_HLO];
-    __shared__ XzItem queue[4][XZ_QCAP];
+    __shared__ XzItem<T> queue[4][XZ_QCAP];
     __shared__ double s_odds[XZ_SPT];
     __shared__ double s_inv[BQ_INV_N];
     const int lane = threadIdx.x & 63;
@@ -478,7 +479,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __syncthreads();
     if (i >= c.N) return;
     uint32_t* hw = hist[wv];
-    XzItem* qw = queue[wv];
+    XzItem<T>* qw = queue[wv];
     hw[lane] = 0u;
     const int G = c.G, t = *c.iter;
     const int b = c.batch[i], k = c.W[i];
@@ -493,7 +494,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     auto zs = c.zs;
     const int z0 = c.zrow[i], z1 = c.zrow[i + 1];
     const uint64_t below = (1ull << lane) - 1ull;
-    uint64_t n1 = 0, sx1 = 0;
+    // Z = 1 entries drawn here and the sum of their values: at most G entries per cell,
+    // each below 256, so 32 bits hold the wave's totals
+    uint32_t n1 = 0, sx1 = 0;
     int nt = 0;
     const int stripe = blockIdx.x % BQ_STRIPES;
     auto tl = c.xz_tail + z0;
@@ -507,8 +510,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         bool tail = false;
         int tp = 0;
         if (lane < take) {
-            const XzItem it = qw[idx];
-            const int g = it.g;
+            const XzItem<T> it = qw[idx];
+            const int g = (int)(it.pg >> 16), ip = z0 + (int)(it.pg & 0xFFFFu);
             const int xo = it.s < 0 ? ~it.s : 0;
             // the stream as bq_xz_update leaves it after the Z draw (three
             // buffered words) or, without a Z draw, before its first block
@@ -517,21 +520,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             r.c3 = ((uint32_t)BQ_PH_XZ << 20) + (xo == 0 ? 1u : 0u);
             r.b0 = it.b0; r.b1 = it.b1; r.b2 = it.b2; r.b3 = 0u; r.n = xo == 0 ? 3 : 0;
             const double u = bq_uniform(&r);
-            if (xo < XZ_SPT && bq_nb_inv_ok(it.mu, it.phi)) {
+            const double mu = (double)it.mu;
+            if (xo < XZ_SPT && bq_nb_inv_ok(mu, it.phi)) {
                 // the inversion returns x' <= 255: both acceptance factors are table entries
-                const int xp = bq_rnbinom_inv_t(u, it.mu, it.phi, s_inv);
+                const int xp = bq_rnbinom_inv_t(u, mu, it.phi, s_inv);
                 const double eo = own ? s_odds[xo] : odt[xo];
                 const double ep = own ? s_odds[xp] : odt[xp];
                 const int xi = bq_uniform(&r) * ep <= eo ? xp : xo;
                 const int32_t sn = ~xi;
-                if (sn != it.s) { row[g] = sn; zs[it.p] = sn; }
+                if (sn != it.s) { row[g] = sn; zs[ip] = sn; }
                 n1 += 1;
-                sx1 += (uint64_t)xi;
+                sx1 += (uint32_t)xi;
                 if (xi < BQ_HLO) atomicAdd(&hw[xi], 1u);
                 else gatomic_add(&c.hist_d[b * H + xi], 1u);
             } else {
                 tail = true;
-                tp = it.p;
+                tp = ip;
             }
         }
         const uint64_t tm = __ballot(tail);
@@ -568,12 +572,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             const uint64_t mlo = __ballot(zq && !hq), mhi = __ballot(hq);
             const uint64_t m0 = __ballot(act && !z);
             if (zq) {
-                XzItem it;
-                it.p = p;
-                it.g = gc;
+                XzItem<T> it;
+                it.pg = (uint32_t)(p - z0) | ((uint32_t)gc << 16);
                 it.s = sc;
                 it.b0 = r.b0; it.b1 = r.b1; it.b2 = r.b2;
-                it.mu = (double)mu;
+                it.mu = mu;
                 it.phi = ph;
                 qw[hq ? XZ_QCAP - 1 - (nhi + __popcll(mhi & below)) : nlo + __popcll(mlo & below)] = it;
             }
@@ -594,8 +597,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         gc = gn;
         sc = sn_next;
     }
-    n1 = wave_sum_u64(n1);
-    sx1 = wave_sum_u64(sx1);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        n1 += __shfl_xor(n1, m, 64);
+        sx1 += __shfl_xor(sx1, m, 64);
+    }
     wave_lds_sync();
     {
         const uint32_t v = hw[lane];
