@@ -456,9 +456,10 @@ static_assert(BQ_HLO == 64, "k_xz keeps one low histogram bin per lane");
 #define XZ_QCAP 192  // queue entries per wave: two stacks of < 128 entries, fewer than 192 together
 #define XZ_MU_SPLIT 6.0  // proposals with a larger mean go to the high stack
 
-// five waves per SIMD: 32-byte queue entries keep the block's LDS under a fifth of the CU's
+// five waves per SIMD in single precision: 32-byte queue entries keep the block's LDS
+// under a fifth of the CU's (four in double precision, with 40-byte entries)
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_xz(const ChainDev* __restrict__ cds)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_xz(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ uint32_t hist[4][BQ_HLO];
@@ -585,15 +586,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             nlo += __popcll(mlo);
             nhi += __popcll(mhi);
         }
-        // full batches of either stack (the newest 64 entries of it)
-        if (nlo >= 64) { flush(64, nlo - 64 + lane); nlo -= 64; }
-        if (nhi >= 64) { flush(64, XZ_QCAP - nhi + lane); nhi -= 64; }
-        if (!more) {
-            // what remains: one batch when it fits, else the high stack first
-            if (nlo + nhi > 64) { flush(nhi, XZ_QCAP - nhi + lane); nhi = 0; }
-            if (nlo + nhi > 0) flush(nlo + nhi, lane < nlo ? lane : XZ_QCAP - nhi + (lane - nlo));
-            break;
+        // Full batches of either stack (the newest 64 entries of it); after the
+        // last round what remains: one batch when it fits, else the high stack
+        // first.  One call site, so that the proposal code is emitted once.
+        for (;;) {
+            int take, idx;
+            if (nlo >= 64) { take = 64; idx = nlo - 64 + lane; nlo -= 64; }
+            else if (nhi >= 64) { take = 64; idx = XZ_QCAP - nhi + lane; nhi -= 64; }
+            else if (!more && nlo + nhi > 64) { take = nhi; idx = XZ_QCAP - nhi + lane; nhi = 0; }
+            else if (!more && nlo + nhi > 0) {
+                take = nlo + nhi;
+                idx = lane < nlo ? lane : XZ_QCAP - nhi + (lane - nlo);
+                nlo = nhi = 0;
+            } else break;
+            flush(take, idx);
         }
+        if (!more) break;
         gc = gn;
         sc = sn_next;
     }
