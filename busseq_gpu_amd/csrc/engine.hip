@@ -631,13 +631,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 // and added once per block.
 // ---------------------------------------------------------------------------
 #define TAIL_CPW 8
+// batches a block's 4 * TAIL_CPW consecutive cells can span (every batch has a cell)
+#define TAIL_NB (4 * TAIL_CPW < BQ_BMAX ? 4 * TAIL_CPW : BQ_BMAX)
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_xz_tail(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    __shared__ unsigned long long st[BQ_BMAX * 5];
-    __shared__ uint32_t hl[BQ_BMAX * BQ_HLO];
+    __shared__ unsigned long long st[TAIL_NB * 5];
+    __shared__ uint32_t hl[TAIL_NB * BQ_HLO];
     __shared__ int incl[4][TAIL_CPW];
     const int N = c.N, G = c.G, B = c.B, t = *c.iter;
     const int cb0 = blockIdx.x * 4 * TAIL_CPW;
