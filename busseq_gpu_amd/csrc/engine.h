@@ -48,6 +48,8 @@
     Q void *ea, *eb, *bt, *en, *ed, *ph;  /* bt: [G][bt_stride(K)][2] (beta_k, expm1 beta_k), k = 1..K-1 */ \
     /* proposals and their precision-T constants */ \
     Q double *alpha_p, *beta_p, *nu_p, *phi_p, *gamma_p; \
+    Q double *delta_p, *delta_lu;  /* [N] delta proposal and log of its accept uniform (k_prep) */ \
+    Q void *dd, *cd;  /* [N] delta' - delta and expm1 of it in precision T */ \
     Q void *da, *ca, *db, *cb, *dn, *cn; \
     Q void* pc;  /* [6][B*G]: phi', d, d/phi, 1/phi', log(phi'/phi), lgamma diff */ \
     Q void* tm;  /* [K*B*G] (e^alpha e^beta_k) e^nu_b in precision T, the X/Z step's mean without e^delta */ \

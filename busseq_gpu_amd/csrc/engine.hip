@@ -199,6 +199,20 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
         c.odds_tab[j] = bq_xz_odds(c.gamma[b], c.gamma[c.B + b], (double)x);
     }
     if (j < 2) c.lstat[j] = 0;
+    if (j < c.N) {
+        // delta proposal and the log of its accept uniform (cu:739-744, 774): one
+        // thread per cell here instead of every lane of k_delta's wave; delta is
+        // not changed before k_delta reads these
+        bq_rng r;
+        bq_rng_init(&r, c.seed, c.key, (uint32_t)j, 0u, (uint32_t)t, BQ_PH_DELTA_P);
+        const double del = c.delta[j];
+        const double dp = del + BQ_RW_SD * bq_normal(&r);
+        const double dl = dp - del;
+        c.delta_p[j] = dp;
+        ((T*)c.dd)[j] = (T)dl;
+        ((T*)c.cd)[j] = (T)bq_expm1(dl);
+        c.delta_lu[j] = bq_log(ph_uniform(c, (uint32_t)j, 0u, t, BQ_PH_DELTA_A));
+    }
     if (j < c.K * c.B * c.G) {
         // ((e^alpha e^beta) e^nu): the first two products of the mean, in its order
         const int G = c.G, g = j % G, kb = j / G, b = kb % c.B, k = kb / c.B;
@@ -1142,7 +1156,6 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
     if (i >= c.N) return;
     const int b = c.batch[i];
     if (i == c.boff[b]) return;
-    const int t = *c.iter;
     const int G = c.G, k = c.W[i];
     const BQ_GLOBAL T* ea = (const BQ_GLOBAL T*)c.ea;
     const BQ_GLOBAL T* eb = (const BQ_GLOBAL T*)c.eb + (size_t)k * G;
@@ -1150,12 +1163,8 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
     const BQ_GLOBAL T* ph = (const BQ_GLOBAL T*)c.ph + (size_t)b * G;
     BQ_GLOBAL T* ed = (BQ_GLOBAL T*)c.ed;
     const T edi = ed[i];
-    const double del = c.delta[i];
-    bq_rng r;
-    bq_rng_init(&r, c.seed, c.key, (uint32_t)i, 0u, (uint32_t)t, BQ_PH_DELTA_P);
-    const double dp = del + BQ_RW_SD * bq_normal(&r);
-    const double dl = dp - del;
-    const T dd = (T)dl, cd = (T)bq_expm1(dl);
+    const double del = c.delta[i], dp = c.delta_p[i];
+    const T dd = ((const BQ_GLOBAL T*)c.dd)[i], cd = ((const BQ_GLOBAL T*)c.cd)[i];
     auto row = c.S + (size_t)i * c.Gp;
     uint64_t a = 0;
     for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
@@ -1188,7 +1197,7 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
     a = wave_sum_u64(a);
     double sum = bq_unfix(a);
     double lr = sum + bq_lr_normal(del, dp, c.mu_delta[i], BQ_SIGMA2_DELTA);
-    if (bq_log(ph_uniform(c, (uint32_t)i, 0u, t, BQ_PH_DELTA_A)) <= lr) {
+    if (c.delta_lu[i] <= lr) {
         if (lane == 0) {
             c.delta[i] = dp;
             ed[i] = (T)bq_exp(dp);
@@ -1786,6 +1795,8 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.alpha_p = s->dalloc<double>(G); c.beta_p = s->dalloc<double>((size_t)K * G);
             c.nu_p = s->dalloc<double>((size_t)B * G); c.phi_p = s->dalloc<double>((size_t)B * G);
             c.gamma_p = s->dalloc<double>(2 * (size_t)B);
+            c.delta_p = s->dalloc<double>(N); c.delta_lu = s->dalloc<double>(N);
+            c.dd = s->dalloc<char>(ts * N); c.cd = s->dalloc<char>(ts * N);
             c.da = s->dalloc<char>(ts * G); c.ca = s->dalloc<char>(ts * G);
             c.db = s->dalloc<char>(ts * K * G); c.cb = s->dalloc<char>(ts * K * G);
             c.dn = s->dalloc<char>(ts * B * G); c.cn = s->dalloc<char>(ts * B * G);
@@ -1853,7 +1864,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             if (rc != BUSSEQ_OK) return fail(rc);
             auto mx = [](int& a, long v) { if (v > a) a = (int)v; };
             mx(s->maxN, N); mx(s->maxG, G); mx(s->maxK, K); mx(s->maxB, B); mx(s->maxT, T_max);
-            mx(s->maxPrep, G); mx(s->maxPrep, (long)K * B * G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * BQ_HLO * BQ_STRIPES); mx(s->maxPrep, (long)B * K * BQ_STRIPES);
+            mx(s->maxPrep, G); mx(s->maxPrep, N); mx(s->maxPrep, (long)K * B * G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * BQ_HLO * BQ_STRIPES); mx(s->maxPrep, (long)B * K * BQ_STRIPES);
             const long nA = (G + 63) / 64;
             mx(s->maxAdec, nA + nblk((long)G * ((K - 1) + (B - 1) + B)));
             mx(s->maxDec, nA * (K - 1)); mx(s->maxDec, nA * (B - 1)); mx(s->maxDec, nA * B);
