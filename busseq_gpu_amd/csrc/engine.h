@@ -59,7 +59,10 @@
     Q int32_t* tiles;  /* [T_max*4] (b, k, start, count) */ \
     Q int32_t* tile_off;  /* [B*K+1] first tile of each (batch, subtype) */ \
     Q int32_t* ntiles;  /* [1] */ \
-    Q uint64_t* part;  /* [T_max][3][Gp] exact tile partial sums */ \
+    Q uint64_t* gsum;  /* exact per-item sums the tile passes add their tile values to (wrapping \
+                        * 64-bit fixed point, so any order gives the same bits); read and cleared \
+                        * by the decision kernels: alpha [G] | beta [K*G] | nu without and with \
+                        * the accepted beta of the group's subtype [2][B*K*G] | phi [B*G] */ \
     /* gamma statistics */ \
     Q uint32_t* hist_d;  /* [B*H] dynamic histogram (entries with Y = 0), values >= BQ_HLO */ \
     Q uint32_t* hist_lo;  /* [BQ_STRIPES][B*BQ_HLO] the same for values < BQ_HLO, striped */ \

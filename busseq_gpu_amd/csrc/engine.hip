@@ -27,9 +27,11 @@
 // precision and the (chunk, lane) sums are added exactly in fixed point.
 // Tile passes: a workgroup owns one tile (<= TR cells of one (batch, subtype)
 // group, in stable sorted order) and 256 genes; each lane sums its gene over
-// the tile in double precision and stores the exact fixed-point value.  The
-// decision kernels add tile values exactly.  The host oracle uses the same
-// grouping, so the sampler reproduces it bit for bit.
+// the tile in double precision and adds the exact fixed-point value to the
+// item's sum with a 64-bit atomic (wrapping integer adds: the order of the
+// tiles does not change a bit); the decision kernels read and clear the sums.
+// The host oracle uses the same grouping, so the sampler reproduces it bit
+// for bit.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -68,6 +70,26 @@ __device__ __forceinline__ float ttphi(int xi, float x, float ph, float pp, floa
 { return bq_tphi_f32(xi, x, ph, pp, mu, d, dop, ip, lr, dl); }
 __device__ __forceinline__ double ttphi(int xi, double x, double ph, double pp, double mu, double d, double dop, double ip, double lr, double dl)
 { return bq_tphi_f64(xi, x, ph, pp, mu, d, dop, ip, lr, dl); }
+
+// sections of a chain's exact per-item sums (gsum)
+__host__ __device__ __forceinline__ size_t gs_beta(int G) { return (size_t)G; }
+__host__ __device__ __forceinline__ size_t gs_nu(int G, int K) { return (size_t)G + (size_t)K * G; }
+__host__ __device__ __forceinline__ size_t gs_phi(int G, int K, int B) { return gs_nu(G, K) + 2 * (size_t)B * K * G; }
+__host__ __device__ __forceinline__ size_t gs_size(int G, int K, int B) { return gs_phi(G, K, B) + (size_t)B * G; }
+
+// a tile value added to an exact per-item sum (no return value: nothing waits on it)
+__device__ __forceinline__ void gsum_add(BQ_GLOBAL uint64_t* p, uint64_t v)
+{
+    if (v) atomicAdd((unsigned long long*)p, (unsigned long long)v);
+}
+
+// an exact per-item sum, read and cleared for the next sweep
+__device__ __forceinline__ uint64_t gsum_take(BQ_GLOBAL uint64_t* p)
+{
+    const uint64_t v = *p;
+    *p = 0;
+    return v;
+}
 
 // value of type V at byte offset o (32-bit) from a wave-uniform base: the form
 // the compiler emits as a global load with a scalar base and a lane offset
@@ -776,40 +798,11 @@ __global__ __launch_bounds__(256) void k_alpha(const ChainDev* __restrict__ cds)
         const T rt = tratio(mu, phg);
         acc += (double)ttmh((T)xi, phg, rt, dag, cag);
     });
-    c.part[((size_t)tile * 3) * c.Gp + g] = bq_fix(acc);
+    gsum_add(c.gsum + g, bq_fix(acc));
 }
 
-// exact sum of the fixed-point tile partials p[m * stride], m in [0, n), with
-// eight loads in flight (wrapping integer adds: the order does not matter)
-__device__ __forceinline__ uint64_t sum_tiles_n(const uint64_t* p, int n, size_t stride)
-{
-    uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int m = 0;
-    for (; m + 8 <= n; m += 8) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] += p[(size_t)(m + u) * stride];
-    }
-    for (; m < n; ++m) a[0] += p[(size_t)m * stride];
-    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-}
-
-// The decision kernels run one block per 64 consecutive columns (genes) of an
-// item class; the block's four waves split the tiles of each column and add
-// their partials in LDS.  sum_tiles_w is wave w's share of the tiles [t0, t1):
-// t0 + w, t0 + w + 4, ...
-__device__ __forceinline__ uint64_t sum_tiles_w(const uint64_t* p, int t0, int t1, size_t stride, int w)
-{
-    const int n = t1 - t0 > w ? (t1 - t0 - w + 3) / 4 : 0;
-    return n ? sum_tiles_n(p + (size_t)(t0 + w) * stride, n, 4 * stride) : 0;
-}
-
-// the four waves' partials of each column added; valid in wave 0
-__device__ __forceinline__ uint64_t combine_waves(uint64_t a, uint64_t (*sp)[64], int wv, int lane)
-{
-    sp[wv][lane] = a;
-    __syncthreads();
-    return (sp[0][lane] + sp[1][lane]) + (sp[2][lane] + sp[3][lane]);
-}
+// The decision kernels run one block per 64 consecutive items (genes) of an
+// item class; its first wave takes the items' exact sums (gsum) and decides.
 
 // nu and phi proposals of k_alpha_dec: item j of G(B-1) (g, b>=1) nu items
 // followed by G B (g, b) phi items
@@ -862,18 +855,15 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    __shared__ uint64_t s_p[4][64];
     __shared__ uint64_t s_r3[3][4];
     const int G = c.G, K = c.K;
     const int nA = (G + 63) / 64;
     const int t = *c.iter;
     if ((int)blockIdx.x < nA) {
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        const int g = blockIdx.x * 64 + lane, nt = *c.ntiles;
-        const uint64_t a = g < G ? sum_tiles_w((const uint64_t*)(c.part + g), 0, nt, (size_t)3 * c.Gp, wv) : 0;
-        const uint64_t tot = combine_waves(a, s_p, wv, lane);
+        const int g = blockIdx.x * 64 + lane;
         if (wv != 0 || g >= G) return;
-        double sum = bq_unfix(tot);
+        double sum = bq_unfix(gsum_take(c.gsum + g));
         double al = c.alpha[g], ap = c.alpha_p[g];
         double lr = sum + bq_lr_normal(al, ap, c.mu_alpha[g], BQ_SIGMA2_ALPHA);
         if (bq_log(ph_uniform(c, (uint32_t)g, 0u, t, BQ_PH_ALPHA_A)) <= lr) {
@@ -1060,10 +1050,13 @@ __global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cd
             if (k >= 1) pn1 += (double)ttmh2(x, phg, rt, dng, cbn, lb);
         }
     });
-    auto pt = c.part + (size_t)tile * 3 * c.Gp;
-    pt[g] = bq_fix(pb);
-    pt[c.Gp + g] = bq_fix(pn0);
-    pt[2 * (size_t)c.Gp + g] = bq_fix(pn1);
+    const int K = c.K;
+    const size_t bkg = ((size_t)b * K + k) * G + g, BKG = (size_t)c.B * K * G;
+    if (k >= 1) gsum_add(c.gsum + gs_beta(G) + kg, bq_fix(pb));
+    if (b >= 1) {
+        gsum_add(c.gsum + gs_nu(G, K) + bkg, bq_fix(pn0));
+        if (k >= 1) gsum_add(c.gsum + gs_nu(G, K) + BKG + bkg, bq_fix(pn1));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1075,23 +1068,14 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    __shared__ uint64_t s_p[4][64];
-    const int G = c.G, K = c.K, B = c.B;
+    const int G = c.G, K = c.K;
     const int nA = (G + 63) / 64;
     if ((int)blockIdx.x >= nA * (K - 1)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int k = 1 + blockIdx.x / nA, g = (blockIdx.x % nA) * 64 + lane, t = *c.iter;
-    const size_t Gp = c.Gp;
-    auto toff = c.tile_off;
-    uint64_t a = 0;
-    if (g < G)
-        for (int b = wv; b < B; b += 4)
-            a += sum_tiles_n((const uint64_t*)(c.part + (size_t)toff[b * K + k] * 3 * Gp + g),
-                             toff[b * K + k + 1] - toff[b * K + k], 3 * Gp);
-    a = combine_waves(a, s_p, wv, lane);
     if (wv != 0 || g >= G) return;
     const size_t q = (size_t)k * G + g;
-    double sum = bq_unfix(a);
+    double sum = bq_unfix(gsum_take(c.gsum + gs_beta(G) + q));
     double be = c.beta[q], bp = c.beta_p[q];
     double lr = sum + bq_lr_beta(be, bp, c.L[q], c.ptau[1]);
     int acc = bq_log(ph_uniform(c, (uint32_t)g, (uint32_t)k, t, BQ_PH_BETA_A)) <= lr;
@@ -1115,23 +1099,20 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_n_dec(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    __shared__ uint64_t s_p[4][64];
     const int G = c.G, K = c.K, B = c.B;
     const int nA = (G + 63) / 64;
     if ((int)blockIdx.x >= nA * (B - 1)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int b = 1 + blockIdx.x / nA, g = (blockIdx.x % nA) * 64 + lane, t = *c.iter;
-    const size_t Gp = c.Gp;
-    auto toff = c.tile_off;
-    uint64_t a = 0;
-    if (g < G)
-        for (int k = wv; k < K; k += 4) {
-            const int slot = (k >= 1 && c.beta_acc[(size_t)k * G + g]) ? 2 : 1;
-            a += sum_tiles_n((const uint64_t*)(c.part + ((size_t)toff[b * K + k] * 3 + slot) * Gp + g),
-                             toff[b * K + k + 1] - toff[b * K + k], 3 * Gp);
-        }
-    a = combine_waves(a, s_p, wv, lane);
     if (wv != 0 || g >= G) return;
+    // each subtype group's nu sum under its accepted beta branch; both cleared
+    const size_t BKG = (size_t)B * K * G;
+    BQ_GLOBAL uint64_t* ns = c.gsum + gs_nu(G, K) + (size_t)b * K * G + g;
+    uint64_t a = 0;
+    for (int k = 0; k < K; ++k) {
+        const uint64_t a0 = gsum_take(ns + (size_t)k * G), a1 = gsum_take(ns + BKG + (size_t)k * G);
+        a += (k >= 1 && c.beta_acc[(size_t)k * G + g]) ? a1 : a0;
+    }
     const size_t q = (size_t)b * G + g;
     double sum = bq_unfix(a);
     double nv = c.nu[q], np = c.nu_p[q];
@@ -1231,7 +1212,7 @@ __global__ __launch_bounds__(256) void k_phi(const ChainDev* __restrict__ cds)
         const T mu = tmu(eag, ebg, eng, edi);
         acc += (double)ttphi(xi, (T)xi, phg, fp, mu, d, dop, ip, lrat, dlg0);
     });
-    c.part[((size_t)tile * 3) * c.Gp + g] = bq_fix(acc);
+    gsum_add(c.gsum + gs_phi(G, c.K, c.B) + bg, bq_fix(acc));
 }
 
 // ---------------------------------------------------------------------------
@@ -1242,18 +1223,14 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_phi_dec(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    __shared__ uint64_t s_p[4][64];
     const int G = c.G, K = c.K, B = c.B;
     const int nA = (G + 63) / 64;
     if ((int)blockIdx.x >= nA * B) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int b = blockIdx.x / nA, g = (blockIdx.x % nA) * 64 + lane, t = *c.iter;
-    const size_t Gp = c.Gp;
-    uint64_t a = g < G ? sum_tiles_w((const uint64_t*)(c.part + g), c.tile_off[b * K], c.tile_off[(b + 1) * K], 3 * Gp, wv) : 0;
-    a = combine_waves(a, s_p, wv, lane);
     if (wv != 0 || g >= G) return;
     const size_t q = (size_t)b * G + g;
-    double sum = bq_unfix(a);
+    double sum = bq_unfix(gsum_take(c.gsum + gs_phi(G, K, B) + q));
     double f = c.phi[q], fp = c.phi_p[q];
     double u = ph_uniform(c, (uint32_t)g, (uint32_t)b, t, BQ_PH_PHI_A);
     if (!(fp > 0.0)) return;
@@ -1807,7 +1784,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.tiles = s->dalloc<int32_t>((size_t)T_max * 4);
             c.tile_off = s->dalloc<int32_t>((size_t)B * K + 1);
             c.ntiles = s->dalloc<int32_t>(1);
-            c.part = s->dalloc<uint64_t>((size_t)T_max * 3 * Gp);
+            c.gsum = s->dalloc<uint64_t>(gs_size(G, K, B));
             c.hist_d = s->dalloc<uint32_t>((size_t)B * H);
             c.hist_lo = s->dalloc<uint32_t>((size_t)B * BQ_HLO * BQ_STRIPES);
             c.gstat = s->dalloc<uint64_t>((size_t)B * 5 * BQ_STRIPES);
