@@ -2,25 +2,25 @@
 // that allocates the chains, captures the sweep into a hipGraph and replays it.
 //
 // Sweep (one hipGraph, fifteen launches, chains on blockIdx.z):
-//   k_prep       alpha/gamma proposals, X/Z tables, zero the per-sweep counters
+//   k_prep       alpha/gamma/delta proposals, X/Z tables, zero the per-sweep counters
+//   k_draws      L draws; beta, nu, phi proposals
 //   k_w          W_i: categorical full conditional, one wave per cell     [row pass]
 //   k_plan       stable sort of cells by (b, W), tiles
 //   k_xz         X/Z imputation of the Y = 0 entries, gamma histogram     [row pass]
 //   k_xz_tail    the X/Z entries with a gamma-Poisson proposal
 //   k_alpha      alpha partial sums                                      [tile pass]
-//   k_alpha_dec  alpha accept, L draw, beta/nu/phi proposals
+//   k_alpha_dec  alpha accept
 //   k_gamma_ptau gamma0/gamma1 accept per batch; pi_b Dirichlet draw; p, tau0 draw
 //   k_beta_nu    beta and nu partial sums (nu under both beta outcomes)  [tile pass]
 //   k_b_dec      beta accept, per (gene, subtype)
 //   k_n_dec      nu accept with the accepted beta branch, per (gene, batch)
-//   k_delta      delta proposal, row sum and accept, one wave per cell   [row pass]
+//   k_delta      delta row sum and accept, one wave per cell             [row pass]
 //   k_phi        phi partial sums with the new delta                     [tile pass]
-//   k_phi_dec    phi accept
-//   k_record     posterior accumulators and traces after burn-in; subtype
-//                counts cleared; iteration counter (by the last block)
-// Three kernels have no data dependence on the launch before them and run on
-// a second branch of the graph: k_prep beside k_w, k_plan beside k_xz and
-// k_xz_tail, k_gamma_ptau beside k_beta_nu.
+//   k_record     phi accept; posterior accumulators and traces after burn-in;
+//                subtype counts cleared; iteration counter (by the last block)
+// Four kernels have no data dependence on the launches beside them and run on
+// a second branch of the graph: k_prep and k_draws beside k_w, k_plan beside
+// k_xz and k_xz_tail, k_gamma_ptau beside k_beta_nu.
 //
 // Row passes: genes are cut into chunks of BQ_CHUNK = 1024; lane l of the wave
 // owning cell i sums the genes g = l (mod 64) of each chunk in double
@@ -845,37 +845,46 @@ __device__ __forceinline__ void dec_proposals(const ChainG& c, int j, int t)
 }
 
 // ---------------------------------------------------------------------------
-// k_alpha_dec: the first ceil(G/64) blocks decide alpha (cu:625-632), 64 genes
-// each; the blocks after them take flat work items
-//   G(K-1)  (g, k>=1)    L Gibbs draw (cu:634-648), beta proposal (cu:670-674)
-//   + G(B-1)  (g, b>=1)  nu proposal (cu:709-713)
-//   + G B     (g, b)     phi proposal (cu:780-787) and the phi-term constants
+// k_alpha_dec: alpha accept (cu:625-632), block j deciding genes [64 j, 64 j + 64)
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = blockIdx.x * 64 + lane;
+    if (wv != 0 || g >= c.G) return;
+    const int t = *c.iter;
+    double sum = bq_unfix(gsum_take(c.gsum + g));
+    double al = c.alpha[g], ap = c.alpha_p[g];
+    double lr = sum + bq_lr_normal(al, ap, c.mu_alpha[g], BQ_SIGMA2_ALPHA);
+    if (bq_log(ph_uniform(c, (uint32_t)g, 0u, t, BQ_PH_ALPHA_A)) <= lr) {
+        c.alpha[g] = ap;
+        ((T*)c.ea)[g] = (T)bq_exp(ap);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_draws: flat work items drawn ahead of their passes, beside the subtype draw
+// (they read beta, nu, phi, p and tau0, which no kernel changes before their
+// first reader, k_beta_nu; the Philox counters are those of their step):
+//   G(K-1)  (g, k>=1)    L Gibbs draw (cu:634-648), beta proposal (cu:670-674)
+//   G(B-1)  (g, b>=1)    nu proposal (cu:709-713)
+//   G B     (g, b)       phi proposal and its per-(b,g) constants (cu:780-787)
+// Blocks holding L items add sum L and sum beta^2 over L = 0 for the p and tau0
+// draws (cu:654-662) to lstat, which k_prep clears before this kernel runs.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_draws(const ChainDev* __restrict__ cds)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ uint64_t s_r3[3][4];
     const int G = c.G, K = c.K;
-    const int nA = (G + 63) / 64;
     const int t = *c.iter;
-    if ((int)blockIdx.x < nA) {
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        const int g = blockIdx.x * 64 + lane;
-        if (wv != 0 || g >= G) return;
-        double sum = bq_unfix(gsum_take(c.gsum + g));
-        double al = c.alpha[g], ap = c.alpha_p[g];
-        double lr = sum + bq_lr_normal(al, ap, c.mu_alpha[g], BQ_SIGMA2_ALPHA);
-        if (bq_log(ph_uniform(c, (uint32_t)g, 0u, t, BQ_PH_ALPHA_A)) <= lr) {
-            c.alpha[g] = ap;
-            ((T*)c.ea)[g] = (T)bq_exp(ap);
-        }
-        return;
-    }
-    int j = (blockIdx.x - nA) * 256 + threadIdx.x;
+    int j = blockIdx.x * 256 + threadIdx.x;
     // blocks holding L items add sum L and sum beta^2 over L = 0 for the p and
     // tau0 draws (cu:654-662), with the new L and the beta before its update
-    const bool lblk = (int)(blockIdx.x - nA) * 256 < G * (K - 1);
+    const bool lblk = (int)blockIdx.x * 256 < G * (K - 1);
     uint64_t lsum = 0, bsq = 0;
     if (j < G * (K - 1)) {
         const int k = 1 + j / G, g = j % G;
@@ -906,6 +915,7 @@ __global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ 
         }
     }
 }
+
 
 // ---------------------------------------------------------------------------
 // k_gamma_ptau: gamma0 then gamma1 accept per batch from the X histogram
@@ -1216,33 +1226,8 @@ __global__ __launch_bounds__(256) void k_phi(const ChainDev* __restrict__ cds)
 }
 
 // ---------------------------------------------------------------------------
-// k_phi_dec: phi accept (cu:806-818), one item per (g, b); block
-// b * ceil(G/64) + j takes genes [64 j, 64 j + 64) of batch b
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_phi_dec(const ChainDev* __restrict__ cds)
-{
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    const int G = c.G, K = c.K, B = c.B;
-    const int nA = (G + 63) / 64;
-    if ((int)blockIdx.x >= nA * B) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int b = blockIdx.x / nA, g = (blockIdx.x % nA) * 64 + lane, t = *c.iter;
-    if (wv != 0 || g >= G) return;
-    const size_t q = (size_t)b * G + g;
-    double sum = bq_unfix(gsum_take(c.gsum + gs_phi(G, K, B) + q));
-    double f = c.phi[q], fp = c.phi_p[q];
-    double u = ph_uniform(c, (uint32_t)g, (uint32_t)b, t, BQ_PH_PHI_A);
-    if (!(fp > 0.0)) return;
-    double lr = sum + bq_lr_phi(f, fp);
-    if (bq_log(u) <= lr) {
-        c.phi[q] = fp;
-        ((T*)c.ph)[q] = (T)fp;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// k_record: posterior accumulators and traces (cu:892-936, 1547-1554)
+// k_record: phi accept, then posterior accumulators and traces (cu:892-936,
+// 1547-1554)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void record_sweep(const ChainG& c, int t, int j, int G, int K, int B, int N, int P)
 {
@@ -1307,12 +1292,32 @@ __device__ __forceinline__ void record_done(const ChainG& c)
     }
 }
 
+// phi accept (cu:806-818) of item j = b G + g
+template <typename T>
+__device__ __forceinline__ void phi_decide(const ChainG& c, int j, int t)
+{
+    const int G = c.G, B = c.B, K = c.K;
+    const int b = j / G, g = j - b * G;
+    double sum = bq_unfix(gsum_take(c.gsum + gs_phi(G, K, B) + j));
+    double f = c.phi[j], fp = c.phi_p[j];
+    double u = ph_uniform(c, (uint32_t)g, (uint32_t)b, t, BQ_PH_PHI_A);
+    if (!(fp > 0.0)) return;
+    double lr = sum + bq_lr_phi(f, fp);
+    if (bq_log(u) <= lr) {
+        c.phi[j] = fp;
+        ((T*)c.ph)[j] = (T)fp;
+    }
+}
+
+// Thread j decides phi item j (the last step of the sweep), then records.
+template <typename T>
 __global__ __launch_bounds__(256) void k_record(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int t = *c.iter;
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int G = c.G, K = c.K, B = c.B, N = c.N, P = c.n_pres;
+    if (j < B * G) phi_decide<T>(c, j, t);
     if (j < B * K * BQ_STRIPES) c.cnt[j] = 0;
     if (t >= c.n_burnin) record_sweep(c, t, j, G, K, B, N, P);
     record_done(c);
@@ -1377,7 +1382,7 @@ struct busseq_sampler {
     int C = 0;                     // chains
     int prec = 32, TR = 64, dev = 0;
     int maxN = 0, maxG = 0, maxK = 0, maxB = 0, maxT = 0, maxPrep = 0, maxRec = 0;
-    int maxAdec = 0, maxDec = 0;   // blocks of k_alpha_dec and of the other decision kernels
+    int maxDraws = 0, maxDec = 0;  // blocks of k_draws and of the per-item decision kernels
     std::vector<ChainHost> ch;
     std::vector<ChainDev> hd;
     ChainDev* d_cd = nullptr;
@@ -1406,8 +1411,8 @@ struct busseq_sampler {
 namespace {
 
 static const char* const kKernelNames[] = {
-    "k_prep", "k_w", "k_plan", "k_xz", "k_xz_tail", "k_alpha", "k_alpha_dec", "k_gamma_ptau",
-    "k_beta_nu", "k_b_dec", "k_n_dec", "k_delta", "k_phi", "k_phi_dec", "k_record"};
+    "k_prep", "k_draws", "k_w", "k_plan", "k_xz", "k_xz_tail", "k_alpha", "k_alpha_dec", "k_gamma_ptau",
+    "k_beta_nu", "k_b_dec", "k_n_dec", "k_delta", "k_phi", "k_record"};
 static const int kNumKernels = 15;
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
@@ -1424,7 +1429,8 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     const dim3 g_rows((s->maxN + 3) / 4, 1, C);
     const dim3 g_plan(s->maxB, 1, C);
     const dim3 g_tile(nblk(s->maxG), s->maxT, C);
-    const dim3 g_adec(s->maxAdec, 1, C);
+    const dim3 g_adec((s->maxG + 63) / 64, 1, C);
+    const dim3 g_draws(s->maxDraws, 1, C);
     const dim3 g_dec(s->maxDec, 1, C);
     const dim3 g_gamma(2 * s->maxB + 1, 1, C);
     const dim3 g_rec(nblk(s->maxRec), 1, C);
@@ -1436,17 +1442,18 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     hipStream_t sd = fork ? s->side : st;
 #define FORK(e) do { if (fork) { HIPCHK(hipEventRecord(s->fev[e], st)); HIPCHK(hipStreamWaitEvent(sd, s->fev[e], 0)); } } while (0)
 #define JOIN(e) do { if (fork) { HIPCHK(hipEventRecord(s->fev[e], sd)); HIPCHK(hipStreamWaitEvent(st, s->fev[e], 0)); } } while (0)
-    // k_prep beside k_w: it reads none of W and the subtype counts, the only
-    // things k_w writes, and k_w reads none of its outputs
+    // k_prep and k_draws beside k_w: they read none of W and the subtype
+    // counts, the only things k_w writes, and k_w reads none of their outputs
     FORK(0);
     EV(); hipLaunchKernelGGL(k_prep<T>, g_prep, blk, 0, sd, cd);
+    EV(); hipLaunchKernelGGL(k_draws<T>, g_draws, blk, 0, sd, cd);
     EV();
     for (const auto& kr : s->krange) {
         int maxn = 0;
         for (int c = kr.first; c < kr.first + kr.second; ++c) maxn = s->ch[c].N > maxn ? s->ch[c].N : maxn;
         launch_w<T>(s->ch[kr.first].K, dim3((maxn + 3) / 4, 1, kr.second), blk, st, cd + kr.first);
     }
-    JOIN(1);   // k_xz reads k_prep's tables and cleared counters
+    JOIN(1);   // k_xz reads k_prep's tables and cleared counters (k_draws' outputs: k_beta_nu on)
     // k_plan (tiles and order of the cells, from W) beside the X/Z kernels,
     // which read neither
     FORK(2);
@@ -1465,8 +1472,7 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     EV(); hipLaunchKernelGGL(k_n_dec<T>, g_dec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_phi_dec<T>, g_dec, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_record, g_rec, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_record<T>, g_rec, blk, 0, st, cd);
     EV();
 #undef FORK
 #undef JOIN
@@ -1843,7 +1849,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             mx(s->maxN, N); mx(s->maxG, G); mx(s->maxK, K); mx(s->maxB, B); mx(s->maxT, T_max);
             mx(s->maxPrep, G); mx(s->maxPrep, N); mx(s->maxPrep, (long)K * B * G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * BQ_HLO * BQ_STRIPES); mx(s->maxPrep, (long)B * K * BQ_STRIPES);
             const long nA = (G + 63) / 64;
-            mx(s->maxAdec, nA + nblk((long)G * ((K - 1) + (B - 1) + B)));
+            mx(s->maxDraws, nblk((long)G * ((K - 1) + (B - 1) + B)));
             mx(s->maxDec, nA * (K - 1)); mx(s->maxDec, nA * (B - 1)); mx(s->maxDec, nA * B);
             mx(s->maxRec, N); mx(s->maxRec, (long)K * G); mx(s->maxRec, (long)B * G); mx(s->maxRec, (long)B * K);
             mx(s->maxRec, 2L * B); mx(s->maxRec, (long)B * K * BQ_STRIPES);
