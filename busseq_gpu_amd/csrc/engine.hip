@@ -1,7 +1,7 @@
 // engine.hip -- gfx950 kernels of one BUSseq Gibbs sweep and the host driver
 // that allocates the chains, captures the sweep into a hipGraph and replays it.
 //
-// Sweep (one hipGraph, fifteen launches, chains on blockIdx.z):
+// Sweep (one hipGraph, fourteen launches, chains on blockIdx.z):
 //   k_prep       alpha/gamma/delta proposals, X/Z tables, zero the per-sweep counters
 //   k_draws      L draws; beta, nu, phi proposals
 //   k_w          W_i: categorical full conditional, one wave per cell     [row pass]
@@ -12,8 +12,8 @@
 //   k_alpha_dec  alpha accept
 //   k_gamma_ptau gamma0/gamma1 accept per batch; pi_b Dirichlet draw; p, tau0 draw
 //   k_beta_nu    beta and nu partial sums (nu under both beta outcomes)  [tile pass]
-//   k_b_dec      beta accept, per (gene, subtype)
-//   k_n_dec      nu accept with the accepted beta branch, per (gene, batch)
+//   k_bn_dec     beta accept per (gene, subtype), then nu accept with the
+//                accepted beta branch per (gene, batch); a block owns 64 genes
 //   k_delta      delta row sum and accept, one wave per cell             [row pass]
 //   k_phi        phi partial sums with the new delta                     [tile pass]
 //   k_record     phi accept; posterior accumulators and traces after burn-in;
@@ -1070,20 +1070,15 @@ __global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cd
 }
 
 // ---------------------------------------------------------------------------
-// k_b_dec: beta accept (cu:696-707), one item per (g, k >= 1); block
-// (k - 1) * ceil(G/64) + j takes genes [64 j, 64 j + 64) of subtype k, wave w
-// the groups (b, k) with b = w mod 4
+// k_bn_dec: beta accept (cu:696-707) for the items (k >= 1, g), then nu accept
+// (cu:730-737) for the items (b >= 1, g) with the nu sums of each subtype's
+// accepted beta branch.  Block j owns genes [64 j, 64 j + 64): the nu step reads
+// only its own genes' beta outcomes, written before the block barrier.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
+__device__ __forceinline__ void beta_decide(const ChainG& c, int k, int g, int t)
 {
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int G = c.G, K = c.K;
-    const int nA = (G + 63) / 64;
-    if ((int)blockIdx.x >= nA * (K - 1)) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int k = 1 + blockIdx.x / nA, g = (blockIdx.x % nA) * 64 + lane, t = *c.iter;
-    if (wv != 0 || g >= G) return;
     const size_t q = (size_t)k * G + g;
     double sum = bq_unfix(gsum_take(c.gsum + gs_beta(G) + q));
     double be = c.beta[q], bp = c.beta_p[q];
@@ -1099,22 +1094,10 @@ __global__ __launch_bounds__(256) void k_b_dec(const ChainDev* __restrict__ cds)
     }
 }
 
-// ---------------------------------------------------------------------------
-// k_n_dec: nu accept (cu:730-737), one item per (g, b >= 1), using the nu sums
-// of the accepted beta branch of each subtype; block (b - 1) * ceil(G/64) + j
-// takes genes [64 j, 64 j + 64) of batch b, wave w the groups (b, k) with
-// k = w mod 4
-// ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_n_dec(const ChainDev* __restrict__ cds)
+__device__ __forceinline__ void nu_decide(const ChainG& c, int b, int g, int t)
 {
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int G = c.G, K = c.K, B = c.B;
-    const int nA = (G + 63) / 64;
-    if ((int)blockIdx.x >= nA * (B - 1)) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int b = 1 + blockIdx.x / nA, g = (blockIdx.x % nA) * 64 + lane, t = *c.iter;
-    if (wv != 0 || g >= G) return;
     // each subtype group's nu sum under its accepted beta branch; both cleared
     const size_t BKG = (size_t)B * K * G;
     BQ_GLOBAL uint64_t* ns = c.gsum + gs_nu(G, K) + (size_t)b * K * G + g;
@@ -1131,6 +1114,21 @@ __global__ __launch_bounds__(256) void k_n_dec(const ChainDev* __restrict__ cds)
         c.nu[q] = np;
         ((T*)c.en)[q] = (T)bq_exp(np);
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bn_dec(const ChainDev* __restrict__ cds)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    const int G = c.G, K = c.K, B = c.B;
+    if ((int)blockIdx.x * 64 >= G) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = blockIdx.x * 64 + lane, t = *c.iter;
+    for (int k = 1 + wv; k < K; k += 4)
+        if (g < G) beta_decide<T>(c, k, g, t);
+    __syncthreads();
+    for (int b = 1 + wv; b < B; b += 4)
+        if (g < G) nu_decide<T>(c, b, g, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -1382,7 +1380,7 @@ struct busseq_sampler {
     int C = 0;                     // chains
     int prec = 32, TR = 64, dev = 0;
     int maxN = 0, maxG = 0, maxK = 0, maxB = 0, maxT = 0, maxPrep = 0, maxRec = 0;
-    int maxDraws = 0, maxDec = 0;  // blocks of k_draws and of the per-item decision kernels
+    int maxDraws = 0;              // blocks of k_draws
     std::vector<ChainHost> ch;
     std::vector<ChainDev> hd;
     ChainDev* d_cd = nullptr;
@@ -1412,8 +1410,8 @@ namespace {
 
 static const char* const kKernelNames[] = {
     "k_prep", "k_draws", "k_w", "k_plan", "k_xz", "k_xz_tail", "k_alpha", "k_alpha_dec", "k_gamma_ptau",
-    "k_beta_nu", "k_b_dec", "k_n_dec", "k_delta", "k_phi", "k_record"};
-static const int kNumKernels = 15;
+    "k_beta_nu", "k_bn_dec", "k_delta", "k_phi", "k_record"};
+static const int kNumKernels = 14;
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
@@ -1431,7 +1429,6 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     const dim3 g_tile(nblk(s->maxG), s->maxT, C);
     const dim3 g_adec((s->maxG + 63) / 64, 1, C);
     const dim3 g_draws(s->maxDraws, 1, C);
-    const dim3 g_dec(s->maxDec, 1, C);
     const dim3 g_gamma(2 * s->maxB + 1, 1, C);
     const dim3 g_rec(nblk(s->maxRec), 1, C);
     int q = 0;
@@ -1467,9 +1464,8 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     FORK(4);
     EV(); hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, sd, cd);
     EV(); hipLaunchKernelGGL(k_beta_nu<T>, g_tile, blk, 0, st, cd);
-    JOIN(5);   // k_b_dec reads tau0
-    EV(); hipLaunchKernelGGL(k_b_dec<T>, g_dec, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_n_dec<T>, g_dec, blk, 0, st, cd);
+    JOIN(5);   // k_bn_dec reads tau0
+    EV(); hipLaunchKernelGGL(k_bn_dec<T>, g_adec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_record<T>, g_rec, blk, 0, st, cd);
@@ -1848,9 +1844,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             auto mx = [](int& a, long v) { if (v > a) a = (int)v; };
             mx(s->maxN, N); mx(s->maxG, G); mx(s->maxK, K); mx(s->maxB, B); mx(s->maxT, T_max);
             mx(s->maxPrep, G); mx(s->maxPrep, N); mx(s->maxPrep, (long)K * B * G); mx(s->maxPrep, (long)B * H); mx(s->maxPrep, (long)B * BQ_HLO * BQ_STRIPES); mx(s->maxPrep, (long)B * K * BQ_STRIPES);
-            const long nA = (G + 63) / 64;
             mx(s->maxDraws, nblk((long)G * ((K - 1) + (B - 1) + B)));
-            mx(s->maxDec, nA * (K - 1)); mx(s->maxDec, nA * (B - 1)); mx(s->maxDec, nA * B);
             mx(s->maxRec, N); mx(s->maxRec, (long)K * G); mx(s->maxRec, (long)B * G); mx(s->maxRec, (long)B * K);
             mx(s->maxRec, 2L * B); mx(s->maxRec, (long)B * K * BQ_STRIPES);
         }
