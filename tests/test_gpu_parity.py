@@ -138,3 +138,19 @@ def test_raw_means_span_batches_and_gene_slabs():
     Y = bq.simulate(n_b, 300, 3, 17)
     s, o = run_pair(Y, n_b, 3, (1,), 32, tile_rows=16)
     s.close()
+
+
+def test_graph_branches_equal_in_order_launches(small_data):
+    # the captured sweep runs k_prep/k_draws, k_plan and k_gamma_ptau on a second
+    # graph branch; the per-kernel timing path runs every kernel in order on one
+    # stream: both must give the oracle's state bit for bit
+    d = small_data
+    kw = dict(seed=23, n_iter=30, n_burnin=4, n_unchanged=2, precision=32)
+    o = po.Oracle(d["Y"], d["n_b"], d["K"], **kw)
+    o.sweeps(6)
+    with bq.Sampler(d["Y"], d["n_b"], d["K"], **kw) as sg, bq.Sampler(d["Y"], d["n_b"], d["K"], **kw) as ss:
+        sg.sweeps(6)
+        ss.kernel_times(6)
+        assert ss.iteration == sg.iteration == 6
+        compare(sg.state(), o.state(), "graph")
+        compare(ss.state(), o.state(), "in order")
