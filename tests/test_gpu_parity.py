@@ -154,3 +154,15 @@ def test_graph_branches_equal_in_order_launches(small_data):
         assert ss.iteration == sg.iteration == 6
         compare(sg.state(), o.state(), "graph")
         compare(ss.state(), o.state(), "in order")
+
+
+@pytest.mark.parametrize("n_b, G, K, prec", [
+    ([30, 24, 20, 16], 130, 7, 32),                          # the subtype count of N=20000, G=3000, K=7
+    ([30, 24, 20, 16], 130, 7, 64),
+    ([26, 22, 20, 18, 16, 15, 14, 13], 200, 12, 32),         # B=8, K=12 as at N=150000, G=5000
+    ([20, 17, 17], 70, 16, 32),                              # the largest K the sampler takes
+])
+def test_large_k_instantiations_match_oracle(n_b, G, K, prec):
+    Y = bq.simulate(n_b, G, K, 40 + K)
+    s, o = run_pair(Y, n_b, K, (1, 2, 3), prec, tile_rows=8, n_iter=20, n_burnin=4, n_unchanged=2)
+    s.close()
