@@ -94,7 +94,7 @@ BQ_FN T bq_tmh2_##SFX(T x, T phi, T r, T dlt, T c12, T l1)                      
  *   [lgamma(phi'+x)-lgamma(phi+x)] - [lgamma(phi')-lgamma(phi)] - x*log(phi'/phi)  \
  *   + (phi+x)*log1p(d*mu/(phi*(phi'+mu))) - d*log1p(mu/phi'),  d = phi'-phi.       \
  * dop = d/phi, iphip = 1/phi', lrat = log(phi'/phi), dlg0 = lgamma(phi')-lgamma(phi) */ \
-BQ_FN T bq_tphi_##SFX(int xi, T x, T phi, T phip, T mu, T d, T dop, T iphip, T lrat, T dlg0) \
+BQ_FN T bq_tphi_gen_##SFX(int xi, T x, T phi, T phip, T mu, T d, T dop, T iphip, T lrat, T dlg0) \
 {                                                                                   \
     T l1 = LOG1P((dop * mu) / (phip + mu));                                        \
     T l2 = LOG1P(mu * iphip);                                                      \
@@ -105,6 +105,41 @@ BQ_FN T bq_tphi_##SFX(int xi, T x, T phi, T phip, T mu, T d, T dop, T iphip, T l
 
 BQ_DEFINE_ELEMENT_TERMS(float, f32, bq_log1pf, bq_lgamma_diff32, __builtin_fmaf)
 BQ_DEFINE_ELEMENT_TERMS(double, f64, bq_log1p, bq_lgamma_diff64, __builtin_fma)
+
+BQ_FN double bq_tphi_f64(int xi, double x, double phi, double phip, double mu, double d, double dop,
+                         double iphip, double lrat, double dlg0)
+{ return bq_tphi_gen_f64(xi, x, phi, phip, mu, d, dop, iphip, lrat, dlg0); }
+
+/* The single-precision phi term: the expression of bq_tphi_gen_f32 with its two
+ * reciprocals, 1/(phi' + mu) and the Stirling difference's 1/((phi + x)(phi' + x)),
+ * taken from one division of their product (a lane with x = 0, which has no
+ * lgamma difference, uses 1 for the second factor); a lane whose lgamma
+ * arguments need the shift (phi + x < 1 or phi' + x < 1) takes bq_lgamma_diff32. */
+BQ_FN float bq_tphi_f32(int xi, float x, float phi, float phip, float mu, float d, float dop, float iphip,
+                        float lrat, float dlg0)
+{
+    const float a = phip + mu;
+    const float z = phi + x, zp = z + d;
+    const int direct = xi > 0 && z >= 1.0f && zp >= 1.0f;
+    const float zz = direct ? z * zp : 1.0f;
+    const float r = 1.0f / (a * zz);
+    const float l1 = bq_log1pf((dop * mu) * (zz * r));
+    const float l2 = bq_log1pf(mu * iphip);
+    float t = __builtin_fmaf(-d, l2, __builtin_fmaf(z, l1, -(x * lrat)));
+    if (xi > 0) {
+        float dl;
+        if (direct) {
+            const float w = a * r;   /* 1/(z zp) */
+            const float iz = zp * w, izp = z * w;
+            const float core = __builtin_fmaf(d, bq_logf_norm(zp), __builtin_fmaf(z - 0.5f, bq_log1pf(d * iz), -d));
+            dl = core + (bq_stirling_s32(izp) - bq_stirling_s32(iz));
+        } else {
+            dl = bq_lgamma_diff32(z, d);
+        }
+        t = t + (dl - dlg0);
+    }
+    return t;
+}
 
 /* ------------------------------------------------------------------------ */
 /* per-parameter MH terms (double precision)                                  */
