@@ -492,10 +492,10 @@ static_assert(BQ_HLO == 64, "k_xz keeps one low histogram bin per lane");
 #define XZ_QCAP 192  // queue entries per wave: two stacks of < 128 entries, fewer than 192 together
 #define XZ_MU_SPLIT 6.0  // proposals with a larger mean go to the high stack
 
-// five waves per SIMD in single precision: 32-byte queue entries keep the block's LDS
-// under a fifth of the CU's (four in double precision, with 40-byte entries)
+// four waves per SIMD: the registers of two rounds of gathers in flight fit
+// without spills (at five a round's gathers were waited for at its start)
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_xz(const ChainDev* __restrict__ cds)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xz(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ uint32_t hist[4][BQ_HLO];
@@ -580,25 +580,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         nt += __popcll(tm);
         wave_lds_sync();
     };
-    // entries of the current round; the next round's are loaded while this one runs
-    int gc = 0;
-    int32_t sc = 0;
-    if (z0 + lane < z1) { gc = c.zcol[z0 + lane]; sc = zs[z0 + lane]; }
+    // entries of the current round with their mean factor and phi, the next
+    // round's entries, whose gathers are issued while this round runs, and the
+    // entries of the round after it
+    int gc = 0, gn = 0;
+    int32_t sc = 0, sc1 = 0;
+    T mc = (T)0;
+    double ph = 0.0;
+    if (z0 + lane < z1) { gc = c.zcol[z0 + lane]; sc = zs[z0 + lane]; mc = tmk[gc]; ph = phi[gc]; }
+    if (z0 + 64 + lane < z1) { gn = c.zcol[z0 + 64 + lane]; sc1 = zs[z0 + 64 + lane]; }
     for (int base = z0;; base += 64) {
         const bool more = base < z1;
-        int gn = 0;
-        int32_t sn_next = 0;
+        int g2 = 0;
+        int32_t s2 = 0;
+        T mcn = (T)0;
+        double phn = 0.0;
         if (more) {
-            const int pn = base + 64 + lane;
-            if (pn < z1) { gn = c.zcol[pn]; sn_next = zs[pn]; }
+            const int p2 = base + 128 + lane;
+            if (p2 < z1) { g2 = c.zcol[p2]; s2 = zs[p2]; }
+            if (base + 64 + lane < z1) { mcn = tmk[gn]; phn = phi[gn]; }
             const int p = base + lane;
             const bool act = p < z1;
             int z = 0;
             bq_rng r;
-            T mc = (T)0;
-            double ph = 0.0;
             if (act) {
-                mc = tmk[gc]; ph = phi[gc];
                 bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)gc, (uint32_t)t, BQ_PH_XZ);
                 const int xo = sc < 0 ? ~sc : 0;   // X = 0 redraws Z whatever Z was (cu:493-496)
                 z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
@@ -638,8 +643,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
             flush(take, idx);
         }
         if (!more) break;
-        gc = gn;
-        sc = sn_next;
+        gc = gn; sc = sc1; mc = mcn; ph = phn;
+        gn = g2; sc1 = s2;
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
