@@ -1,7 +1,7 @@
 // engine.hip -- gfx950 kernels of one BUSseq Gibbs sweep and the host driver
 // that allocates the chains, captures the sweep into a hipGraph and replays it.
 //
-// Sweep (one hipGraph, fourteen launches, chains on blockIdx.z):
+// Sweep (one hipGraph, thirteen launches, chains on blockIdx.z):
 //   k_prep       alpha/gamma/delta proposals, X/Z tables, zero the per-sweep counters
 //   k_draws      L draws; beta, nu, phi proposals
 //   k_w          W_i: categorical full conditional, one wave per cell     [row pass]
@@ -10,17 +10,18 @@
 //   k_xz_tail    the X/Z entries with a gamma-Poisson proposal
 //   k_alpha      alpha partial sums                                      [tile pass]
 //   k_alpha_dec  alpha accept
-//   k_gamma_ptau gamma0/gamma1 accept per batch; pi_b Dirichlet draw; p, tau0 draw
-//   k_beta_nu    beta and nu partial sums (nu under both beta outcomes)  [tile pass]
+//   k_beta_nu    beta and nu partial sums (nu under both beta outcomes)  [tile pass];
+//                its leading blocks: gamma0/gamma1 accept per batch, pi_b
+//                Dirichlet draw, p and tau0 draw
 //   k_bn_dec     beta accept per (gene, subtype), then nu accept with the
 //                accepted beta branch per (gene, batch); a block owns 64 genes
 //   k_delta      delta row sum and accept, one wave per cell             [row pass]
 //   k_phi        phi partial sums with the new delta                     [tile pass]
 //   k_record     phi accept; posterior accumulators and traces after burn-in;
 //                subtype counts cleared; iteration counter (by the last block)
-// Four kernels have no data dependence on the launches beside them and run on
+// Three kernels have no data dependence on the launches beside them and run on
 // a second branch of the graph: k_prep and k_draws beside k_w, k_plan beside
-// k_xz and k_xz_tail, k_gamma_ptau beside k_beta_nu.
+// k_xz and k_xz_tail.
 //
 // Row passes: genes are cut into chunks of BQ_CHUNK = 1024; lane l of the wave
 // owning cell i sums the genes g = l (mod 64) of each chunk in double
@@ -923,22 +924,24 @@ __global__ __launch_bounds__(256) void k_draws(const ChainDev* __restrict__ cds)
 
 
 // ---------------------------------------------------------------------------
-// k_gamma_ptau: gamma0 then gamma1 accept per batch from the X histogram
+// gamma_ptau_block: gamma0 then gamma1 accept per batch from the X histogram
 // (cu:524-600); pi_b ~ Dirichlet(n_b. + xi) (cu:856-890, normalisation race
-// fixed); p and tau0 (cu:650-668).  Grid: nbx gamma blocks (one per batch),
-// nbx pi blocks, one p/tau0 block.  pi is read only by the next sweep's subtype
-// draw and by the recorder, so it is drawn here, beside the gamma blocks.
+// fixed); p and tau0 (cu:650-668).  Block j of a chain's 2B + 1: j < B the
+// gamma step of batch j, B <= j < 2B the pi draw of batch j - B, j = 2B p and
+// tau0.  These blocks lead the grid of k_beta_nu, which reads none of gamma,
+// pi, p and tau0 (a second graph branch for them cost two cross-queue waits,
+// about as long as the blocks themselves); pi is read only by the next sweep's
+// subtype draw and by the recorder.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__ cds)
+__device__ __forceinline__ void gamma_ptau_block(const ChainG& c, int j)
 {
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ uint64_t s_red[3][4];
     __shared__ int s_cnt[BQ_KMAX];
     __shared__ int64_t s_n[H];        // histogram of X over the batch's entries (static + dynamic)
     __shared__ uint64_t s_gs[5];      // the batch's gamma statistics, stripes added
     const int tid = threadIdx.x, B = c.B, t = *c.iter;
-    const int nbx = (int)(gridDim.x - 1) / 2;
-    if (blockIdx.x == gridDim.x - 1) {
+    if (j > 2 * B) return;
+    if (j == 2 * B) {
         if (t < c.n_unchanged || tid != 0) return;
         const int G = c.G, K = c.K;
         const double n = (double)G * (K - 1);
@@ -951,24 +954,25 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
         c.ptau[1] = 1.0 / bq_rgamma(&r, a, 1.0 / bb);
         return;
     }
-    if ((int)blockIdx.x >= nbx) {
-        const int b = blockIdx.x - nbx, K = c.K;
-        if (b >= B || tid >= 64) return;
+    if (j >= B) {
+        const int b = j - B, K = c.K;
+        if (tid >= 64) return;
         if (tid < K) s_cnt[tid] = sum_stripes<int>((const int32_t*)c.cnt + b * K + tid, (size_t)B * K);
         wave_lds_sync();
         if (tid != 0) return;
         bq_rng r;
         bq_rng_init(&r, c.seed, c.key, (uint32_t)b, 0u, (uint32_t)t, BQ_PH_PI);
-        double gk[BQ_KMAX], tot = 0.0;
+        // the K gamma draws go to pi first and are divided by their sum in place
+        double tot = 0.0;
         for (int k = 0; k < K; ++k) {
-            gk[k] = bq_rgamma(&r, s_cnt[k] + BQ_XI_PI, 1.0);
-            tot += gk[k];
+            const double gk = bq_rgamma(&r, s_cnt[k] + BQ_XI_PI, 1.0);
+            c.pi[b * K + k] = gk;
+            tot += gk;
         }
-        for (int k = 0; k < K; ++k) c.pi[b * K + k] = gk[k] / tot;
+        for (int k = 0; k < K; ++k) c.pi[b * K + k] = c.pi[b * K + k] / tot;
         return;
     }
-    const int b = blockIdx.x;
-    if (b >= B) return;
+    const int b = j;
     auto hs = c.hist_s + (size_t)b * H;
     for (int x = tid; x < H; x += 256) {
         int64_t n = hs[x];
@@ -1033,10 +1037,16 @@ __global__ __launch_bounds__(256) void k_gamma_ptau(const ChainDev* __restrict__
 // is kept for both outcomes of the beta step of the cell's subtype
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_beta_nu(const ChainDev* __restrict__ cds)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 5))) void k_beta_nu(
+    const ChainDev* __restrict__ cds, int gy)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    const int tile = blockIdx.y;
+    // the first gy rows of blocks: the gamma, pi, p and tau0 blocks
+    if ((int)blockIdx.y < gy) {
+        gamma_ptau_block(c, (int)(blockIdx.y * gridDim.x + blockIdx.x));
+        return;
+    }
+    const int tile = blockIdx.y - gy;
     if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= c.G) return;
@@ -1414,9 +1424,9 @@ struct busseq_sampler {
 namespace {
 
 static const char* const kKernelNames[] = {
-    "k_prep", "k_draws", "k_w", "k_plan", "k_xz", "k_xz_tail", "k_alpha", "k_alpha_dec", "k_gamma_ptau",
+    "k_prep", "k_draws", "k_w", "k_plan", "k_xz", "k_xz_tail", "k_alpha", "k_alpha_dec",
     "k_beta_nu", "k_bn_dec", "k_delta", "k_phi", "k_record"};
-static const int kNumKernels = 14;
+static const int kNumKernels = 13;
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
@@ -1434,7 +1444,8 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     const dim3 g_tile(nblk(s->maxG), s->maxT, C);
     const dim3 g_adec((s->maxG + 63) / 64, 1, C);
     const dim3 g_draws(s->maxDraws, 1, C);
-    const dim3 g_gamma(2 * s->maxB + 1, 1, C);
+    // rows of k_beta_nu blocks that carry the 2B + 1 gamma, pi, p/tau0 blocks of each chain
+    const int gy_gamma = (int)((2 * s->maxB + 1 + g_tile.x - 1) / g_tile.x);
     const dim3 g_rec(nblk(s->maxRec), 1, C);
     int q = 0;
 #define EV() do { if (ev) HIPCHK(hipEventRecord(ev[q], st)); ++q; } while (0)
@@ -1465,11 +1476,7 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     JOIN(3);   // k_alpha reads the tiles
     EV(); hipLaunchKernelGGL(k_alpha<T>, g_tile, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_adec, blk, 0, st, cd);
-    // k_gamma_ptau (gamma, p, tau0, pi) beside k_beta_nu, which reads none of them
-    FORK(4);
-    EV(); hipLaunchKernelGGL(k_gamma_ptau, g_gamma, blk, 0, sd, cd);
-    EV(); hipLaunchKernelGGL(k_beta_nu<T>, g_tile, blk, 0, st, cd);
-    JOIN(5);   // k_bn_dec reads tau0
+    EV(); hipLaunchKernelGGL(k_beta_nu<T>, dim3(g_tile.x, g_tile.y + gy_gamma, C), blk, 0, st, cd, gy_gamma);
     EV(); hipLaunchKernelGGL(k_bn_dec<T>, g_adec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);

@@ -141,7 +141,7 @@ def test_raw_means_span_batches_and_gene_slabs():
 
 
 def test_graph_branches_equal_in_order_launches(small_data):
-    # the captured sweep runs k_prep/k_draws, k_plan and k_gamma_ptau on a second
+    # the captured sweep runs k_prep/k_draws and k_plan on a second
     # graph branch; the per-kernel timing path runs every kernel in order on one
     # stream: both must give the oracle's state bit for bit
     d = small_data
