@@ -1,27 +1,28 @@
 // engine.hip -- gfx950 kernels of one BUSseq Gibbs sweep and the host driver
 // that allocates the chains, captures the sweep into a hipGraph and replays it.
 //
-// Sweep (one hipGraph, thirteen launches, chains on blockIdx.z):
-//   k_prep       alpha/gamma/delta proposals, X/Z tables, zero the per-sweep counters
-//   k_draws      L draws; beta, nu, phi proposals
+// Sweep (one hipGraph of ten launches in one stream, chains on blockIdx.z):
 //   k_w          W_i: categorical full conditional, one wave per cell     [row pass]
-//   k_plan       stable sort of cells by (b, W), tiles
-//   k_xz         X/Z imputation of the Y = 0 entries, gamma histogram     [row pass]
+//   k_xz         leading blocks: stable sort of cells by (b, W) and tiles (one per
+//                batch); L draws, beta, nu, phi proposals.  Then the X/Z imputation
+//                of the Y = 0 entries and the gamma histogram             [row pass]
 //   k_xz_tail    the X/Z entries with a gamma-Poisson proposal
 //   k_alpha      alpha partial sums                                      [tile pass]
 //   k_alpha_dec  alpha accept
-//   k_beta_nu    beta and nu partial sums (nu under both beta outcomes)  [tile pass];
-//                its leading blocks: gamma0/gamma1 accept per batch, pi_b
-//                Dirichlet draw, p and tau0 draw
+//   k_beta_nu    leading blocks: gamma0/gamma1 accept per batch, pi_b Dirichlet
+//                draw, p and tau0 draw.  Then beta and nu partial sums (nu under
+//                both beta outcomes)                                     [tile pass]
 //   k_bn_dec     beta accept per (gene, subtype), then nu accept with the
 //                accepted beta branch per (gene, batch); a block owns 64 genes
 //   k_delta      delta row sum and accept, one wave per cell             [row pass]
-//   k_phi        phi partial sums with the new delta                     [tile pass]
+//   k_phi        leading blocks: the next sweep's alpha, gamma and delta
+//                proposals, X/Z tables and cleared counters.  Then phi partial
+//                sums with the new delta                                 [tile pass]
 //   k_record     phi accept; posterior accumulators and traces after burn-in;
 //                subtype counts cleared; iteration counter (by the last block)
-// Three kernels have no data dependence on the launches beside them and run on
-// a second branch of the graph: k_prep and k_draws beside k_w, k_plan beside
-// k_xz and k_xz_tail.
+// Work that the launch it joins neither reads nor writes for runs in that
+// launch's leading blocks, not on a second graph branch: a branch's fork and
+// join each cost a cross-queue wait of about 10 us.
 //
 // Row passes: genes are cut into chunks of BQ_CHUNK = 1024; lane l of the wave
 // owning cell i sums the genes g = l (mod 64) of each chunk in double
@@ -184,16 +185,18 @@ __device__ __forceinline__ double ph_uniform(const ChainG& c, uint32_t a, uint32
 }
 
 // ---------------------------------------------------------------------------
-// k_prep: alpha and gamma proposals (cu:514-522, 602-606), counters to zero,
-// table of the X/Z acceptance factors 1 + e^{-(gamma0 + gamma1 x)}, x < H,
-// and the X/Z step's per-(subtype, batch, gene) mean factors
+// prep_item: what sweep t draws and tabulates ahead of its passes, item j:
+// alpha and gamma proposals (cu:514-522, 602-606), delta proposals, counters
+// to zero, the table of the X/Z acceptance factors 1 + e^{-(gamma0 + gamma1 x)},
+// x < H, and the X/Z step's per-(subtype, batch, gene) mean factors.  It reads
+// only parameters final at the end of sweep t - 1, and nothing after k_delta
+// in that sweep reads what it writes, so sweep t - 1's k_phi runs it in its
+// leading blocks; k_prep runs it once for the first sweep after the chain's
+// parameters were set.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
+__device__ __forceinline__ void prep_item(const ChainG& c, int j, int t)
 {
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    const int t = *c.iter;
     if (j < c.G) {
         bq_rng r;
         bq_rng_init(&r, c.seed, c.key, (uint32_t)j, 0u, (uint32_t)t, BQ_PH_ALPHA_P);
@@ -241,6 +244,13 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
         const int G = c.G, g = j % G, kb = j / G, b = kb % c.B, k = kb / c.B;
         ((T*)c.tm)[j] = (((const T*)c.ea)[g] * ((const T*)c.eb)[(size_t)k * G + g]) * ((const T*)c.en)[(size_t)b * G + g];
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    prep_item<T>(c, blockIdx.x * 256 + threadIdx.x, *c.iter);
 }
 
 // ---------------------------------------------------------------------------
@@ -342,20 +352,21 @@ void launch_w(int K, dim3 grid, dim3 blk, hipStream_t st, const ChainDev* cd)
 }
 
 // ---------------------------------------------------------------------------
-// k_plan: stable partition of batch b's cells by subtype (the order every
+// plan_block: stable partition of batch b's cells by subtype (the order every
 // tile pass reads them in) and the tile list.  One block of PLAN_THREADS
-// threads per batch; thread t owns a contiguous run of the batch's cells.
+// threads per batch (the leading blocks of k_xz, which reads neither);
+// thread t owns a contiguous run of the batch's cells.  lds: PLAN_LDS_INTS
+// ints of the caller's shared memory.
 // ---------------------------------------------------------------------------
-#define PLAN_THREADS 1024
-__global__ __launch_bounds__(PLAN_THREADS) void k_plan(const ChainDev* __restrict__ cds)
+#define PLAN_THREADS 256
+#define PLAN_LDS_INTS (2 * BQ_BMAX * BQ_KMAX + 1 + (PLAN_THREADS / 64) * BQ_KMAX)
+__device__ __forceinline__ void plan_block(const ChainG& c, int b, int* lds)
 {
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    const int b = blockIdx.x;
     if (b >= c.B) return;
     const int K = c.K, B = c.B, TR = c.TR, tid = threadIdx.x;
-    __shared__ int s_toff[BQ_BMAX * BQ_KMAX + 1];
-    __shared__ int s_cnt[BQ_BMAX * BQ_KMAX];
-    __shared__ int s_wsum[PLAN_THREADS / 64][BQ_KMAX];
+    int* s_toff = lds;                                   // BQ_BMAX * BQ_KMAX + 1
+    int* s_cnt = lds + BQ_BMAX * BQ_KMAX + 1;            // BQ_BMAX * BQ_KMAX
+    int (*s_wsum)[BQ_KMAX] = (int (*)[BQ_KMAX])(s_cnt + BQ_BMAX * BQ_KMAX);
     for (int j = tid; j < B * K; j += PLAN_THREADS) {
         s_cnt[j] = sum_stripes<int>((const int32_t*)c.cnt + j, (size_t)B * K);
     }
@@ -496,22 +507,41 @@ static_assert(BQ_HLO == 64, "k_xz keeps one low histogram bin per lane");
 // four waves per SIMD: the registers of two rounds of gathers in flight fit
 // without spills (at five a round's gathers were waited for at its start)
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xz(const ChainDev* __restrict__ cds)
+__device__ void draws_block(const ChainG& c, int blk, uint64_t (*s_r3)[4]);
+
+static_assert(sizeof(XzItem<float>) * 4 * XZ_QCAP >= sizeof(int) * PLAN_LDS_INTS,
+              "the plan blocks of k_xz use its queue's shared memory");
+
+// Leading blocks: nplan plan blocks (one per batch), then ndraws blocks of the
+// draws ahead of the parameter passes; neither reads what the X/Z blocks
+// write, nor the other way round.  The rest: four cells per block.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xz(
+    const ChainDev* __restrict__ cds, int nplan, int ndraws)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ uint32_t hist[4][BQ_HLO];
     __shared__ XzItem<T> queue[4][XZ_QCAP];
     __shared__ double s_odds[XZ_SPT];
     __shared__ double s_inv[BQ_INV_N];
+    if ((int)blockIdx.x < nplan) {
+        plan_block(c, (int)blockIdx.x, (int*)&queue[0][0]);
+        return;
+    }
+    if ((int)blockIdx.x < nplan + ndraws) {
+        draws_block<T>(c, (int)blockIdx.x - nplan, (uint64_t (*)[4])&queue[0][0]);
+        return;
+    }
+    const int bx = (int)blockIdx.x - nplan - ndraws;
     const int lane = threadIdx.x & 63;
     // wave and cell are wave-uniform: everything derived from them stays in scalar registers
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = blockIdx.x * 4 + wv;
+    const int i = bx * 4 + wv;
     // the grid covers the largest chain of a batch: blocks past this chain's cells leave
-    if (blockIdx.x * 4 >= c.N) return;
+    if (bx * 4 >= c.N) return;
     // the acceptance-factor table of the batch of the block's first cell; a wave whose cell
     // lies in another batch reads its own batch's table from memory
-    const int bb = c.batch[blockIdx.x * 4];
+    const int bb = c.batch[bx * 4];
     for (int x = threadIdx.x; x < XZ_SPT; x += 256) s_odds[x] = c.odds_tab[(size_t)bb * H + x];
     for (int x = threadIdx.x; x < BQ_INV_N; x += 256) s_inv[x] = bq_inv_tab[x];
     __syncthreads();
@@ -536,7 +566,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // each below 256, so 32 bits hold the wave's totals
     uint32_t n1 = 0, sx1 = 0;
     int nt = 0;
-    const int stripe = blockIdx.x % BQ_STRIPES;
+    const int stripe = bx % BQ_STRIPES;
     auto tl = c.xz_tail + z0;
     // Queued proposals: a batch of 64 runs its CDF search up to the largest
     // draw among its lanes, so proposals are kept in two stacks by their mean:
@@ -871,26 +901,24 @@ __global__ __launch_bounds__(256) void k_alpha_dec(const ChainDev* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// k_draws: flat work items drawn ahead of their passes, beside the subtype draw
-// (they read beta, nu, phi, p and tau0, which no kernel changes before their
-// first reader, k_beta_nu; the Philox counters are those of their step):
+// draws_block: flat work items drawn ahead of their passes, in leading blocks
+// of k_xz (they read beta, nu, phi, p and tau0, which no kernel changes before
+// their first reader, k_beta_nu; the Philox counters are those of their step):
 //   G(K-1)  (g, k>=1)    L Gibbs draw (cu:634-648), beta proposal (cu:670-674)
 //   G(B-1)  (g, b>=1)    nu proposal (cu:709-713)
 //   G B     (g, b)       phi proposal and its per-(b,g) constants (cu:780-787)
 // Blocks holding L items add sum L and sum beta^2 over L = 0 for the p and tau0
-// draws (cu:654-662) to lstat, which k_prep clears before this kernel runs.
+// draws (cu:654-662) to lstat, which prep_item clears ahead of the sweep.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_draws(const ChainDev* __restrict__ cds)
+__device__ __forceinline__ void draws_block(const ChainG& c, int blk, uint64_t (*s_r3)[4])
 {
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    __shared__ uint64_t s_r3[3][4];
     const int G = c.G, K = c.K;
     const int t = *c.iter;
-    int j = blockIdx.x * 256 + threadIdx.x;
+    int j = blk * 256 + threadIdx.x;
     // blocks holding L items add sum L and sum beta^2 over L = 0 for the p and
     // tau0 draws (cu:654-662), with the new L and the beta before its update
-    const bool lblk = (int)blockIdx.x * 256 < G * (K - 1);
+    const bool lblk = blk * 256 < G * (K - 1);
     uint64_t lsum = 0, bsq = 0;
     if (j < G * (K - 1)) {
         const int k = 1 + j / G, g = j % G;
@@ -1210,13 +1238,20 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
 }
 
 // ---------------------------------------------------------------------------
-// k_phi: phi partial sums (cu:789-804) with the new delta
+// k_phi: phi partial sums (cu:789-804) with the new delta.  Its first gy rows
+// of blocks run prep_item for the next sweep (they read and write nothing the
+// phi sums and k_record use).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_phi(const ChainDev* __restrict__ cds)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 5))) void k_phi(
+    const ChainDev* __restrict__ cds, int gy)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    const int tile = blockIdx.y;
+    if ((int)blockIdx.y < gy) {
+        prep_item<T>(c, (int)(blockIdx.y * gridDim.x + blockIdx.x) * 256 + (int)threadIdx.x, *c.iter + 1);
+        return;
+    }
+    const int tile = blockIdx.y - gy;
     if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= c.G) return;
@@ -1395,14 +1430,12 @@ struct busseq_sampler {
     int C = 0;                     // chains
     int prec = 32, TR = 64, dev = 0;
     int maxN = 0, maxG = 0, maxK = 0, maxB = 0, maxT = 0, maxPrep = 0, maxRec = 0;
-    int maxDraws = 0;              // blocks of k_draws
+    int maxDraws = 0;              // draws blocks (leading blocks of k_xz)
     std::vector<ChainHost> ch;
     std::vector<ChainDev> hd;
     ChainDev* d_cd = nullptr;
     std::vector<std::pair<int, int>> krange;   // runs of consecutive chains with equal K
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;    // second branch of the captured sweep (default priority)
-    hipEvent_t fev[6] = {};        // fork and join points of the captured sweep
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<void*> allocs;
@@ -1424,9 +1457,9 @@ struct busseq_sampler {
 namespace {
 
 static const char* const kKernelNames[] = {
-    "k_prep", "k_draws", "k_w", "k_plan", "k_xz", "k_xz_tail", "k_alpha", "k_alpha_dec",
+    "k_w", "k_xz", "k_xz_tail", "k_alpha", "k_alpha_dec",
     "k_beta_nu", "k_bn_dec", "k_delta", "k_phi", "k_record"};
-static const int kNumKernels = 13;
+static const int kNumKernels = 10;
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
@@ -1438,54 +1471,53 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     const int C = s->C;
     hipStream_t st = s->stream;
     const ChainDev* cd = s->d_cd;
-    const dim3 g_prep(nblk(s->maxPrep), 1, C);
     const dim3 g_rows((s->maxN + 3) / 4, 1, C);
-    const dim3 g_plan(s->maxB, 1, C);
     const dim3 g_tile(nblk(s->maxG), s->maxT, C);
     const dim3 g_adec((s->maxG + 63) / 64, 1, C);
-    const dim3 g_draws(s->maxDraws, 1, C);
+    // k_xz: the plan blocks (one per batch) and the draws blocks lead the cells' blocks
+    const dim3 g_xz(s->maxB + s->maxDraws + (s->maxN + 3) / 4, 1, C);
     // rows of k_beta_nu blocks that carry the 2B + 1 gamma, pi, p/tau0 blocks of each chain
     const int gy_gamma = (int)((2 * s->maxB + 1 + g_tile.x - 1) / g_tile.x);
+    // rows of k_phi blocks that carry the next sweep's prep items
+    const int gy_prep = (int)((nblk(s->maxPrep) + g_tile.x - 1) / g_tile.x);
     const dim3 g_rec(nblk(s->maxRec), 1, C);
     int q = 0;
+    // One stream, no branches: a second branch of the captured graph costs a
+    // cross-queue wait of about 10 us at each fork and join, more than the short
+    // kernels it would hide, so their work runs as leading blocks of the passes
+    // beside which it may run.  With ev (per-kernel timing) events bracket each.
 #define EV() do { if (ev) HIPCHK(hipEventRecord(ev[q], st)); ++q; } while (0)
-    // Second branch of the captured sweep.  With ev (per-kernel timing) every
-    // kernel runs in order on one stream, so that the events bracket each.
-    const bool fork = ev == nullptr && s->side != nullptr;
-    hipStream_t sd = fork ? s->side : st;
-#define FORK(e) do { if (fork) { HIPCHK(hipEventRecord(s->fev[e], st)); HIPCHK(hipStreamWaitEvent(sd, s->fev[e], 0)); } } while (0)
-#define JOIN(e) do { if (fork) { HIPCHK(hipEventRecord(s->fev[e], sd)); HIPCHK(hipStreamWaitEvent(st, s->fev[e], 0)); } } while (0)
-    // k_prep and k_draws beside k_w: they read none of W and the subtype
-    // counts, the only things k_w writes, and k_w reads none of their outputs
-    FORK(0);
-    EV(); hipLaunchKernelGGL(k_prep<T>, g_prep, blk, 0, sd, cd);
-    EV(); hipLaunchKernelGGL(k_draws<T>, g_draws, blk, 0, sd, cd);
     EV();
     for (const auto& kr : s->krange) {
         int maxn = 0;
         for (int c = kr.first; c < kr.first + kr.second; ++c) maxn = s->ch[c].N > maxn ? s->ch[c].N : maxn;
         launch_w<T>(s->ch[kr.first].K, dim3((maxn + 3) / 4, 1, kr.second), blk, st, cd + kr.first);
     }
-    JOIN(1);   // k_xz reads k_prep's tables and cleared counters (k_draws' outputs: k_beta_nu on)
-    // k_plan (tiles and order of the cells, from W) beside the X/Z kernels,
-    // which read neither
-    FORK(2);
-    EV(); hipLaunchKernelGGL(k_plan, g_plan, dim3(PLAN_THREADS), 0, sd, cd);
-    EV(); hipLaunchKernelGGL(k_xz<T>, g_rows, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_xz<T>, g_xz, blk, 0, st, cd, s->maxB, s->maxDraws);
     EV(); hipLaunchKernelGGL(k_xz_tail<T>, dim3((s->maxN + 4 * TAIL_CPW - 1) / (4 * TAIL_CPW), 1, C), blk, 0, st, cd);
-    JOIN(3);   // k_alpha reads the tiles
     EV(); hipLaunchKernelGGL(k_alpha<T>, g_tile, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_adec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_beta_nu<T>, dim3(g_tile.x, g_tile.y + gy_gamma, C), blk, 0, st, cd, gy_gamma);
     EV(); hipLaunchKernelGGL(k_bn_dec<T>, g_adec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_phi<T>, g_tile, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_phi<T>, dim3(g_tile.x, g_tile.y + gy_prep, C), blk, 0, st, cd, gy_prep);
     EV(); hipLaunchKernelGGL(k_record<T>, g_rec, blk, 0, st, cd);
     EV();
-#undef FORK
-#undef JOIN
 #undef EV
     HIPCHK(hipGetLastError());
+    return BUSSEQ_OK;
+}
+
+// the draws and tables of the current sweep (prep_item) from the parameters on
+// the device: at creation and after busseq_set_state; each sweep's k_phi
+// prepares the one after it
+busseq_status refresh_prep(busseq_sampler* s)
+{
+    const dim3 g(nblk(s->maxPrep), 1, s->C);
+    if (s->prec == 32) hipLaunchKernelGGL(k_prep<float>, g, dim3(256), 0, s->stream, s->d_cd);
+    else hipLaunchKernelGGL(k_prep<double>, g, dim3(256), 0, s->stream, s->d_cd);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
     return BUSSEQ_OK;
 }
 
@@ -1726,16 +1758,10 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
     s->TR = c0->tile_rows > 0 ? c0->tile_rows : 64;
     s->dev = c0->device;
     if (hipSetDevice(s->dev) != hipSuccess) { bq_set_error("hipSetDevice failed (no GPU?)"); return fail(BUSSEQ_ERR_DEVICE); }
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         bq_set_error("hipStreamCreate failed");
         return fail(BUSSEQ_ERR_DEVICE);
     }
-    for (hipEvent_t& e : s->fev)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            bq_set_error("hipEventCreate failed");
-            return fail(BUSSEQ_ERR_DEVICE);
-        }
     const size_t ts = s->prec == 32 ? 4 : 8;
     const int TR = s->TR;
     for (int p = 0; p < n_probs; ++p) {
@@ -1872,6 +1898,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
         return fail(BUSSEQ_ERR_DEVICE);
     }
     if (hipDeviceSynchronize() != hipSuccess) { bq_set_error("device sync failed"); return fail(BUSSEQ_ERR_DEVICE); }
+    if (refresh_prep(s) != BUSSEQ_OK) return fail(BUSSEQ_ERR_DEVICE);
     *out = s;
     return BUSSEQ_OK;
 }
@@ -1893,9 +1920,6 @@ extern "C" void busseq_destroy(busseq_sampler* s)
     if (s->graph) (void)hipGraphDestroy(s->graph);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->d_cd) (void)hipFree(s->d_cd);
-    for (hipEvent_t e : s->fev)
-        if (e) (void)hipEventDestroy(e);
-    if (s->side) (void)hipStreamDestroy(s->side);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -2162,7 +2186,9 @@ extern "C" busseq_status busseq_set_state(busseq_sampler* s, int ch, const busse
     HIPCHK(d2h(nu, c.nu, (size_t)B * G));
     HIPCHK(d2h(de, c.delta, N));
     HIPCHK(d2h(ph, c.phi, (size_t)B * G));
-    return upload_tcopies(s, ch, al, be, nu, de, ph);
+    busseq_status rc = upload_tcopies(s, ch, al, be, nu, de, ph);
+    if (rc != BUSSEQ_OK) return rc;
+    return refresh_prep(s);
 }
 
 extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result* r)

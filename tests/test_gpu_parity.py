@@ -112,6 +112,20 @@ def test_set_state_roundtrip(small_data):
         s.sweeps(2)
         o.sweeps(2)
         compare(s.state(), o.state(), "after set_state")
+        # a state neither side reached by sweeping: the proposals and tables the
+        # next sweep uses must come from it, not from the last sweep's parameters
+        # (scalings keep the parameters fixed at zero there: beta of subtype 0,
+        # nu of batch 0, delta of each batch's first cell)
+        st = o.state()
+        nu = st["nu"].copy()
+        nu[1:] += 0.0625
+        moved = {"alpha": st["alpha"] - 0.25, "delta": st["delta"] * 0.875, "gamma": st["gamma"] * 0.75,
+                 "nu": nu, "beta": st["beta"] * 0.5}
+        s.set_state(moved)
+        o.set_state(moved)
+        s.sweeps(2)
+        o.sweeps(2)
+        compare(s.state(), o.state(), "after setting a new state")
 
 
 @pytest.mark.slow
@@ -140,10 +154,9 @@ def test_raw_means_span_batches_and_gene_slabs():
     s.close()
 
 
-def test_graph_branches_equal_in_order_launches(small_data):
-    # the captured sweep runs k_prep/k_draws and k_plan on a second
-    # graph branch; the per-kernel timing path runs every kernel in order on one
-    # stream: both must give the oracle's state bit for bit
+def test_graph_equals_in_order_launches(small_data):
+    # the captured sweep and the per-kernel timing path (events between the
+    # launches) must both give the oracle's state bit for bit
     d = small_data
     kw = dict(seed=23, n_iter=30, n_burnin=4, n_unchanged=2, precision=32)
     o = po.Oracle(d["Y"], d["n_b"], d["K"], **kw)
