@@ -1436,8 +1436,8 @@ struct busseq_sampler {
     ChainDev* d_cd = nullptr;
     std::vector<std::pair<int, int>> krange;   // runs of consecutive chains with equal K
     hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph[2] = {};      // one sweep, BQ_GRAPH_SWEEPS sweeps
+    hipGraphExec_t gexec[2] = {};
     std::vector<void*> allocs;
     int iter = 0;
     bool oom = false;              // some dalloc failed
@@ -1521,18 +1521,42 @@ busseq_status refresh_prep(busseq_sampler* s)
     return BUSSEQ_OK;
 }
 
+// Graphs of one sweep and of BQ_GRAPH_SWEEPS sweeps: consecutive graph
+// launches are a few microseconds apart on the device, consecutive kernels of
+// one graph are not, so long runs replay the longer graph.
+#define BQ_GRAPH_SWEEPS 4
 busseq_status build_graph(busseq_sampler* s)
 {
-    if (s->gexec) return BUSSEQ_OK;
-    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    busseq_status rc = s->prec == 32 ? enqueue_sweep<float>(s) : enqueue_sweep<double>(s);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(s->stream, &g);
-    if (rc != BUSSEQ_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess) { bq_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return BUSSEQ_ERR_DEVICE; }
-    s->graph = g;
-    HIPCHK(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+    if (s->gexec[0]) return BUSSEQ_OK;
+    for (int v = 0; v < 2; ++v) {
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        busseq_status rc = BUSSEQ_OK;
+        for (int r = 0; r < (v ? BQ_GRAPH_SWEEPS : 1) && rc == BUSSEQ_OK; ++r)
+            rc = s->prec == 32 ? enqueue_sweep<float>(s) : enqueue_sweep<double>(s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s->stream, &g);
+        if (rc != BUSSEQ_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) { bq_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return BUSSEQ_ERR_DEVICE; }
+        s->graph[v] = g;
+        HIPCHK(hipGraphInstantiate(&s->gexec[v], g, nullptr, nullptr, 0));
+    }
     return BUSSEQ_OK;
+}
+
+// n sweeps of every chain, enqueued on the sampler's stream
+busseq_status launch_sweeps(busseq_sampler* s, int n)
+{
+    for (; n >= BQ_GRAPH_SWEEPS; n -= BQ_GRAPH_SWEEPS) HIPCHK(hipGraphLaunch(s->gexec[1], s->stream));
+    for (; n > 0; --n) HIPCHK(hipGraphLaunch(s->gexec[0], s->stream));
+    return BUSSEQ_OK;
+}
+
+void drop_graphs(busseq_sampler* s)
+{
+    for (int v = 0; v < 2; ++v) {
+        if (s->gexec[v]) { (void)hipGraphExecDestroy(s->gexec[v]); s->gexec[v] = nullptr; }
+        if (s->graph[v]) { (void)hipGraphDestroy(s->graph[v]); s->graph[v] = nullptr; }
+    }
 }
 
 template <typename T>
@@ -1916,8 +1940,7 @@ extern "C" void busseq_destroy(busseq_sampler* s)
     if (!s) return;
     (void)hipSetDevice(s->dev);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
+    drop_graphs(s);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->d_cd) (void)hipFree(s->d_cd);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1933,7 +1956,8 @@ extern "C" busseq_status busseq_sweeps(busseq_sampler* s, int n)
     HIPCHK(hipSetDevice(s->dev));
     busseq_status rc = build_graph(s);
     if (rc != BUSSEQ_OK) return rc;
-    for (int j = 0; j < n; ++j) HIPCHK(hipGraphLaunch(s->gexec, s->stream));
+    rc = launch_sweeps(s, n);
+    if (rc != BUSSEQ_OK) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
     s->iter += n;
     return BUSSEQ_OK;
@@ -1950,7 +1974,8 @@ extern "C" busseq_status busseq_time_sweeps(busseq_sampler* s, int n, double* ms
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipEventRecord(e0, s->stream));
-    for (int j = 0; j < n; ++j) HIPCHK(hipGraphLaunch(s->gexec, s->stream));
+    rc = launch_sweeps(s, n);
+    if (rc != BUSSEQ_OK) return rc;
     HIPCHK(hipEventRecord(e1, s->stream));
     HIPCHK(hipEventSynchronize(e1));
     float f = 0.f;
@@ -1984,8 +2009,7 @@ static busseq_status set_tile_rows(busseq_sampler* s, int tr)
     s->TR = tr;
     s->maxT = maxT;
     HIPCHK(hipMemcpy(s->d_cd, s->hd.data(), sizeof(ChainDev) * s->C, hipMemcpyHostToDevice));
-    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
-    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    drop_graphs(s);
     return BUSSEQ_OK;
 }
 
