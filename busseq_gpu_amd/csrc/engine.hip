@@ -706,8 +706,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // batches a block's 4 * TAIL_CPW consecutive cells can span (every batch has a cell)
 #define TAIL_NB (4 * TAIL_CPW < BQ_BMAX ? 4 * TAIL_CPW : BQ_BMAX)
 
+// three waves per SIMD (a few registers spilled) instead of the two that the
+// full update's registers would allow: the waves wait on their double-precision
+// chains, and a third per SIMD hides more of that latency
 template <typename T>
-__global__ __launch_bounds__(256) void k_xz_tail(const ChainDev* __restrict__ cds)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_xz_tail(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ unsigned long long st[TAIL_NB * 5];
