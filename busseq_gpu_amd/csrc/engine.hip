@@ -696,15 +696,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // ---------------------------------------------------------------------------
-// k_xz_tail: the entries k_xz left to the gamma-Poisson proposal.  A wave
-// takes TAIL_CPW consecutive cells and runs over the concatenation of their
-// listed entries with full lanes; the complete update bq_xz_update runs from
-// each entry's stream.  Counts and gamma sums are gathered per batch in LDS
-// and added once per block.
+// k_xz_tail: the entries k_xz left to the gamma-Poisson proposal.  A block
+// takes TAIL_CELLS consecutive cells and runs over the concatenation of their
+// listed entries with full lanes, 64 entries per wave in turn; the complete
+// update bq_xz_update runs from each entry's stream.  Counts and gamma sums
+// are gathered per batch in LDS and added once per block.
 // ---------------------------------------------------------------------------
-#define TAIL_CPW 8
-// batches a block's 4 * TAIL_CPW consecutive cells can span (every batch has a cell)
-#define TAIL_NB (4 * TAIL_CPW < BQ_BMAX ? 4 * TAIL_CPW : BQ_BMAX)
+#define TAIL_CELLS 32   // cells per block of k_xz_tail (a power of two, at most 64)
+// batches a block's TAIL_CELLS consecutive cells can span (every batch has a cell)
+#define TAIL_NB (TAIL_CELLS < BQ_BMAX ? TAIL_CELLS : BQ_BMAX)
 
 // three waves per SIMD (a few registers spilled) instead of the two that the
 // full update's registers would allow: the waves wait on their double-precision
@@ -715,33 +715,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     __shared__ unsigned long long st[TAIL_NB * 5];
     __shared__ uint32_t hl[TAIL_NB * BQ_HLO];
-    __shared__ int incl[4][TAIL_CPW];
+    __shared__ int incl[TAIL_CELLS];
     const int N = c.N, G = c.G, B = c.B, t = *c.iter;
-    const int cb0 = blockIdx.x * 4 * TAIL_CPW;
+    const int cb0 = blockIdx.x * TAIL_CELLS;
     if (cb0 >= N) return;
-    const int cb1 = (cb0 + 4 * TAIL_CPW < N ? cb0 + 4 * TAIL_CPW : N) - 1;
+    const int cb1 = (cb0 + TAIL_CELLS < N ? cb0 + TAIL_CELLS : N) - 1;
     const int bmin = c.batch[cb0], nbb = c.batch[cb1] - bmin + 1;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i0 = cb0 + wv * TAIL_CPW;
     const int stripe = blockIdx.x % BQ_STRIPES;
     for (int q = threadIdx.x; q < nbb * 5; q += 256) st[q] = 0ull;
     for (int q = threadIdx.x; q < nbb * BQ_HLO; q += 256) hl[q] = 0u;
-    // inclusive prefix of the listed counts over the wave's cells
-    int v = (lane < TAIL_CPW && i0 + lane < N) ? c.xz_ntail[i0 + lane] : 0;
+    // inclusive prefix of the listed counts over the block's cells
+    if (wv == 0) {
+        int v = (lane < TAIL_CELLS && cb0 + lane < N) ? c.xz_ntail[cb0 + lane] : 0;
 #pragma unroll
-    for (int o = 1; o < TAIL_CPW; o <<= 1) {
-        const int u = __shfl_up(v, o, 64);
-        if (lane >= o) v += u;
+        for (int o = 1; o < TAIL_CELLS; o <<= 1) {
+            const int u = __shfl_up(v, o, 64);
+            if (lane >= o) v += u;
+        }
+        if (lane < TAIL_CELLS) incl[lane] = v;
     }
-    if (lane < TAIL_CPW) incl[wv][lane] = v;
     __syncthreads();
-    const int total = incl[wv][TAIL_CPW - 1];
-    for (int j = lane; j < total; j += 64) {
+    // the block's listed entries in batches of 64, dealt to its waves in turn,
+    // so the waves of a block differ by at most one batch
+    const int total = incl[TAIL_CELLS - 1];
+    for (int j = wv * 64 + lane; j - lane < total; j += 256) {
+        if (j >= total) continue;
+        // m = the number of cells whose prefix ends at or before entry j: its cell
         int m = 0;
 #pragma unroll
-        for (int q = 0; q < TAIL_CPW - 1; ++q) m += incl[wv][q] <= j;
-        const int i = i0 + m;
-        const int l = j - (m ? incl[wv][m - 1] : 0);
+        for (int step = TAIL_CELLS / 2; step >= 1; step >>= 1)
+            if (incl[m + step - 1] <= j) m += step;
+        const int i = cb0 + m;
+        const int l = j - (m ? incl[m - 1] : 0);
         const int pz = c.xz_tail[c.zrow[i] + l];
         const int g = c.zcol[pz];
         const int b = c.batch[i], k = c.W[i];
@@ -1497,7 +1503,7 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
         launch_w<T>(s->ch[kr.first].K, dim3((maxn + 3) / 4, 1, kr.second), blk, st, cd + kr.first);
     }
     EV(); hipLaunchKernelGGL(k_xz<T>, g_xz, blk, 0, st, cd, s->maxB, s->maxDraws);
-    EV(); hipLaunchKernelGGL(k_xz_tail<T>, dim3((s->maxN + 4 * TAIL_CPW - 1) / (4 * TAIL_CPW), 1, C), blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_xz_tail<T>, dim3((s->maxN + TAIL_CELLS - 1) / TAIL_CELLS, 1, C), blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha<T>, g_tile, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_adec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_beta_nu<T>, dim3(g_tile.x, g_tile.y + gy_gamma, C), blk, 0, st, cd, gy_gamma);
