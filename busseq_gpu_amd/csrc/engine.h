@@ -74,7 +74,7 @@
     Q int32_t* beta_acc;  /* [K*G] 1 where this sweep accepted beta_kg */ \
     /* posterior accumulators */ \
     Q int* n_rec; \
-    Q int* rec_done;  /* [1] blocks of k_record finished this sweep */ \
+    Q int* iter_rec;  /* [1] the sweep index k_record uses, copied from iter by k_bn_dec */ \
     Q double *s_gamma, *s_alpha, *s_beta, *s_nu, *s_delta, *s_phi, *s_pi, *s_ptau; \
     Q int32_t *c_L, *c_W; \
     /* traces, iteration-major: [r * n_params + param] */ \

@@ -19,7 +19,7 @@
 //                proposals, X/Z tables and cleared counters.  Then phi partial
 //                sums with the new delta                                 [tile pass]
 //   k_record     phi accept; posterior accumulators and traces after burn-in;
-//                subtype counts cleared; iteration counter (by the last block)
+//                subtype counts cleared; the sweep index advanced (one thread)
 // Work that the launch it joins neither reads nor writes for runs in that
 // launch's leading blocks, not on a second graph branch: a branch's fork and
 // join each cost a cross-queue wait of about 10 us.
@@ -1176,6 +1176,7 @@ __global__ __launch_bounds__(256) void k_bn_dec(const ChainDev* __restrict__ cds
     if ((int)blockIdx.x * 64 >= G) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = blockIdx.x * 64 + lane, t = *c.iter;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *c.iter_rec = t;   // for k_record
     for (int k = 1 + wv; k < K; k += 4)
         if (g < G) beta_decide<T>(c, k, g, t);
     __syncthreads();
@@ -1334,20 +1335,6 @@ __device__ __forceinline__ void record_sweep(const ChainG& c, int t, int j, int 
 }
 
 
-// The subtype counts of this sweep (read by k_plan and the pi draw) are cleared
-// here for the next one.  Every block then counts itself done; the last block
-// of a chain, after all the others have read the iteration index, advances it.
-__device__ __forceinline__ void record_done(const ChainG& c)
-{
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (gatomic_add(c.rec_done, 1) == (int)gridDim.x - 1) {
-            *c.rec_done = 0;
-            *c.iter += 1;
-        }
-    }
-}
 
 // phi accept (cu:806-818) of item j = b G + g
 template <typename T>
@@ -1366,18 +1353,22 @@ __device__ __forceinline__ void phi_decide(const ChainG& c, int j, int t)
     }
 }
 
-// Thread j decides phi item j (the last step of the sweep), then records.
+// Thread j decides phi item j (the last step of the sweep), then records; the
+// subtype counts of this sweep (read by the plan and the pi draw) are cleared
+// for the next one.  The sweep index comes from iter_rec, which k_bn_dec
+// copied from iter, so one thread can advance iter without racing the other blocks'
+// reads (a last-block count would need a device-wide fence in every block).
 template <typename T>
 __global__ __launch_bounds__(256) void k_record(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    const int t = *c.iter;
+    const int t = *c.iter_rec;
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int G = c.G, K = c.K, B = c.B, N = c.N, P = c.n_pres;
     if (j < B * G) phi_decide<T>(c, j, t);
     if (j < B * K * BQ_STRIPES) c.cnt[j] = 0;
     if (t >= c.n_burnin) record_sweep(c, t, j, G, K, B, N, P);
-    record_done(c);
+    if (j == 0) *c.iter = t + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1868,7 +1859,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.odds_tab = s->dalloc<double>((size_t)B * H);
             c.beta_acc = s->dalloc<int32_t>((size_t)K * G);
             c.n_rec = s->dalloc<int>(1);
-            c.rec_done = s->dalloc<int>(1);
+            c.iter_rec = s->dalloc<int>(1);
             c.s_gamma = s->dalloc<double>(2 * (size_t)B); c.s_alpha = s->dalloc<double>(G);
             c.s_beta = s->dalloc<double>((size_t)K * G); c.s_nu = s->dalloc<double>((size_t)B * G);
             c.s_delta = s->dalloc<double>(N); c.s_phi = s->dalloc<double>((size_t)B * G);
