@@ -1124,7 +1124,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 // ---------------------------------------------------------------------------
 // k_bn_dec: beta accept (cu:696-707) for the items (k >= 1, g), then nu accept
 // (cu:730-737) for the items (b >= 1, g) with the nu sums of each subtype's
-// accepted beta branch.  Block j owns genes [64 j, 64 j + 64): the nu step reads
+// accepted beta branch.  Block j owns genes [32 j, 32 j + 32): the nu step reads
 // only its own genes' beta outcomes, written before the block barrier.
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1168,20 +1168,27 @@ __device__ __forceinline__ void nu_decide(const ChainG& c, int b, int g, int t)
     }
 }
 
+#define BN_GENES 32   // genes per block of k_bn_dec: (K - 1) * 32 beta items fit one pass for K <= 9
 template <typename T>
 __global__ __launch_bounds__(256) void k_bn_dec(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     const int G = c.G, K = c.K, B = c.B;
-    if ((int)blockIdx.x * 64 >= G) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int g = blockIdx.x * 64 + lane, t = *c.iter;
+    const int g0 = blockIdx.x * BN_GENES;
+    if (g0 >= G) return;
+    const int t = *c.iter;
     if (blockIdx.x == 0 && threadIdx.x == 0) *c.iter_rec = t;   // for k_record
-    for (int k = 1 + wv; k < K; k += 4)
-        if (g < G) beta_decide<T>(c, k, g, t);
+    // thread i takes the items i, i + 256, ... of the block's (k, gene) and then
+    // (b, gene) items, gene fastest
+    for (int q = threadIdx.x; q < (K - 1) * BN_GENES; q += 256) {
+        const int g = g0 + q % BN_GENES;
+        if (g < G) beta_decide<T>(c, 1 + q / BN_GENES, g, t);
+    }
     __syncthreads();
-    for (int b = 1 + wv; b < B; b += 4)
-        if (g < G) nu_decide<T>(c, b, g, t);
+    for (int q = threadIdx.x; q < (B - 1) * BN_GENES; q += 256) {
+        const int g = g0 + q % BN_GENES;
+        if (g < G) nu_decide<T>(c, 1 + q / BN_GENES, g, t);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1498,7 +1505,7 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     EV(); hipLaunchKernelGGL(k_alpha<T>, g_tile, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_alpha_dec<T>, g_adec, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_beta_nu<T>, dim3(g_tile.x, g_tile.y + gy_gamma, C), blk, 0, st, cd, gy_gamma);
-    EV(); hipLaunchKernelGGL(k_bn_dec<T>, g_adec, blk, 0, st, cd);
+    EV(); hipLaunchKernelGGL(k_bn_dec<T>, dim3((s->maxG + BN_GENES - 1) / BN_GENES, 1, C), blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_phi<T>, dim3(g_tile.x, g_tile.y + gy_prep, C), blk, 0, st, cd, gy_prep);
     EV(); hipLaunchKernelGGL(k_record<T>, g_rec, blk, 0, st, cd);
