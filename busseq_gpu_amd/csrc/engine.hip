@@ -195,7 +195,7 @@ __device__ __forceinline__ double ph_uniform(const ChainG& c, uint32_t a, uint32
 // parameters were set.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void prep_item(const ChainG& c, int j, int t)
+__device__ __forceinline__ void prep_item(const ChainG& c, int j, int t, bool with_tm)
 {
     if (j < c.G) {
         bq_rng r;
@@ -239,8 +239,9 @@ __device__ __forceinline__ void prep_item(const ChainG& c, int j, int t)
         ((T*)c.cd)[j] = (T)bq_expm1(dl);
         c.delta_lu[j] = bq_log(ph_uniform(c, (uint32_t)j, 0u, t, BQ_PH_DELTA_A));
     }
-    if (j < c.K * c.B * c.G) {
+    if (with_tm && j < c.K * c.B * c.G) {
         // ((e^alpha e^beta) e^nu): the first two products of the mean, in its order
+        // (in the sweep, k_bn_dec writes them once the parameters are decided)
         const int G = c.G, g = j % G, kb = j / G, b = kb % c.B, k = kb / c.B;
         ((T*)c.tm)[j] = (((const T*)c.ea)[g] * ((const T*)c.eb)[(size_t)k * G + g]) * ((const T*)c.en)[(size_t)b * G + g];
     }
@@ -250,7 +251,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    prep_item<T>(c, blockIdx.x * 256 + threadIdx.x, *c.iter);
+    prep_item<T>(c, blockIdx.x * 256 + threadIdx.x, *c.iter, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1189,6 +1190,17 @@ __global__ __launch_bounds__(256) void k_bn_dec(const ChainDev* __restrict__ cds
         const int g = g0 + q % BN_GENES;
         if (g < G) nu_decide<T>(c, 1 + q / BN_GENES, g, t);
     }
+    __syncthreads();
+    // the mean factors (e^alpha e^beta_k) e^nu_b of the block's genes from the
+    // decided parameters, none of which changes again this sweep: k_delta's means
+    // (times e^delta) and the next sweep's X/Z table
+    const T* ea = (const T*)c.ea;
+    const T* eb = (const T*)c.eb;
+    const T* en = (const T*)c.en;
+    for (int q = threadIdx.x; q < K * B * BN_GENES; q += 256) {
+        const int g = g0 + q % BN_GENES, kb = q / BN_GENES, b = kb % B, k = kb / B;
+        if (g < G) ((T*)c.tm)[(size_t)kb * G + g] = (ea[g] * eb[(size_t)k * G + g]) * en[(size_t)b * G + g];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1206,9 +1218,8 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
     const int b = c.batch[i];
     if (i == c.boff[b]) return;
     const int G = c.G, k = c.W[i];
-    const BQ_GLOBAL T* ea = (const BQ_GLOBAL T*)c.ea;
-    const BQ_GLOBAL T* eb = (const BQ_GLOBAL T*)c.eb + (size_t)k * G;
-    const BQ_GLOBAL T* en = (const BQ_GLOBAL T*)c.en + (size_t)b * G;
+    // (e^alpha e^beta_k) e^nu_b of the cell's group, written by k_bn_dec
+    const BQ_GLOBAL T* tm = (const BQ_GLOBAL T*)c.tm + ((size_t)k * c.B + b) * G;
     const BQ_GLOBAL T* ph = (const BQ_GLOBAL T*)c.ph + (size_t)b * G;
     BQ_GLOBAL T* ed = (BQ_GLOBAL T*)c.ed;
     const T edi = ed[i];
@@ -1222,20 +1233,20 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
         // four genes of the lane class per step, loaded together, added in gene order
         for (unsigned g0 = (unsigned)(c0 + lane); g0 < (unsigned)c1; g0 += 256) {
             int32_t sv[4];
-            T pv[4], av[4], bv[4], nv[4];
+            T pv[4], mv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const unsigned g = g0 + 64u * u, o = g * (unsigned)sizeof(T);
                 if (g < (unsigned)c1) {
                     sv[u] = ld_at<int32_t>(row, g * 4u);
-                    pv[u] = ld_at<T>(ph, o); av[u] = ld_at<T>(ea, o); bv[u] = ld_at<T>(eb, o); nv[u] = ld_at<T>(en, o);
+                    pv[u] = ld_at<T>(ph, o); mv[u] = ld_at<T>(tm, o);
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (g0 + 64 * u < c1) {
                     const int xi = sv[u] < 0 ? ~sv[u] : sv[u];
-                    const T mu = tmu(av[u], bv[u], nv[u], edi);
+                    const T mu = mv[u] * edi;   // tmu's last product
                     const T rt = tratio(mu, pv[u]);
                     part += (double)ttmh((T)xi, pv[u], rt, dd, cd);
                 }
@@ -1265,7 +1276,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
     if ((int)blockIdx.y < gy) {
-        prep_item<T>(c, (int)(blockIdx.y * gridDim.x + blockIdx.x) * 256 + (int)threadIdx.x, *c.iter + 1);
+        prep_item<T>(c, (int)(blockIdx.y * gridDim.x + blockIdx.x) * 256 + (int)threadIdx.x, *c.iter + 1, false);
         return;
     }
     const int tile = blockIdx.y - gy;
