@@ -92,7 +92,7 @@ def cpu_baseline(Y, cfg, precision, budget_s):
 
 def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
     """c4: K-sweep K=3..10 with 4 chains each (32 chains) on the c1 data in one
-    sampler (every chain in the same launches of one hipGraph) against the same
+    sampler (every chain in the same kernel launches) against the same
     32 chains run one after another in the plain form."""
     cfg = CONFIGS["c1"]
     n_b, G = cfg["n_b"], cfg["G"]
@@ -143,7 +143,7 @@ def c5_datasets(n=16):
 
 
 def heterogeneous_batch(bq, precision, sweeps=20, warmup=3):
-    """c5: the 16 data sets in one batched sampler (one graph launch per sweep)
+    """c5: the 16 data sets in one batched sampler (every data set in the same launches)
     against the same 16 chains run one after another in the plain form."""
     sets = c5_datasets()
     probs = []
@@ -185,6 +185,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--tile-rows", type=int, default=0,
+                    help="cells per tile of the plain run (0: the autotuner's choice); "
+                         "fixed, it keeps separate profiler passes on the same tiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -204,10 +207,14 @@ def main():
     calib_ms = bq.copy_probe(1 << 30)
     log(f"[bench] copy probe 1 GiB: {calib_ms:.3f} ms = {2 * (1 << 30) / calib_ms / 1e6:.0f} GB/s (read+write)")
 
-    tr, _ = bq.autotune(Y, n_b, K, seed=13579, precision=args.precision, n_burnin=0, n_unchanged=0)
+    if args.tile_rows > 0:
+        tr, how = args.tile_rows, "given"
+    else:
+        tr, _ = bq.autotune(Y, n_b, K, seed=13579, precision=args.precision, n_burnin=0, n_unchanged=0)
+        how = "autotuned"
     t0 = time.perf_counter()
     s = bq.Sampler(Y, n_b, K, seed=13579, precision=args.precision, tile_rows=tr, **PHASE)
-    log(f"[bench] init {time.perf_counter() - t0:.1f}s, tile rows {tr} (autotuned)")
+    log(f"[bench] init {time.perf_counter() - t0:.1f}s, tile rows {tr} ({how})")
     s.sweeps(args.warmup)
     w0 = time.perf_counter()
     ms = s.time_sweeps(args.steps)
