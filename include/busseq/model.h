@@ -80,7 +80,7 @@ BQ_FN T bq_mu_##SFX(T ea, T eb, T en, T ed) { return ((ea * eb) * en) * ed; }   
 /* r = mu / (phi + mu) */                                                          \
 BQ_FN T bq_ratio_##SFX(T mu, T phi) { return mu / (phi + mu); }                    \
 /* subtype-k logit relative to subtype 0: x*beta_k - (phi+x)*log1p(r0*expm1(beta_k)) */ \
-BQ_FN T bq_tw_##SFX(T x, T phi, T r0, T beta, T em)                                \
+BQ_FN T bq_tw_gen_##SFX(T x, T phi, T r0, T beta, T em)                            \
 { return FMA(-(phi + x), LOG1P(r0 * em), x * beta); }                             \
 /* random-walk MH term for a log-mean shift dlt, c = expm1(dlt); l = log1p(r*c) */ \
 BQ_FN T bq_tmh_l_##SFX(T x, T phi, T dlt, T l) { return FMA(-(phi + x), l, x * dlt); } \
@@ -105,6 +105,18 @@ BQ_FN T bq_tphi_gen_##SFX(int xi, T x, T phi, T phip, T mu, T d, T dop, T iphip,
 
 BQ_DEFINE_ELEMENT_TERMS(float, f32, bq_log1pf, bq_lgamma_diff32, __builtin_fmaf)
 BQ_DEFINE_ELEMENT_TERMS(double, f64, bq_log1p, bq_lgamma_diff64, __builtin_fma)
+
+BQ_FN double bq_tw_f64(double x, double phi, double r0, double beta, double em)
+{ return bq_tw_gen_f64(x, phi, r0, beta, em); }
+
+/* The single-precision subtype logit term: expm1(beta) is held to
+ * [-1 + 2^-24, 2^100] (beta beyond about +-69 only), so r0 expm1(beta) with
+ * 0 <= r0 <= 1 stays where the branch-free log1pf holds. */
+BQ_FN float bq_tw_f32(float x, float phi, float r0, float beta, float em)
+{
+    const float emc = __builtin_fminf(__builtin_fmaxf(em, -0.99999994f), 0x1p100f);
+    return __builtin_fmaf(-(phi + x), bq_log1pf_mid(r0 * emc), x * beta);
+}
 
 BQ_FN double bq_tphi_f64(int xi, double x, double phi, double phip, double mu, double d, double dop,
                          double iphip, double lrat, double dlg0)

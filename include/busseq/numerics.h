@@ -381,6 +381,19 @@ BQ_FN float bq_log1pf(float y)
     return res;
 }
 
+/* log1p(y) for y in (-1, sqrt(2) 2^125), and NaN for NaN: bq_log1pf without
+ * its branch to the special values, for callers that keep y in that range */
+BQ_FN float bq_log1pf_mid(float y)
+{
+    float t = 1.0f + y;
+    uint32_t kb = (bq_fbits(t) - 0x3F3504F3u) & 0xFF800000u;
+    float s = bq_bitsf(0x3F800000u - kb);
+    float f = __builtin_fmaf(y, s, s - 1.0f);
+    float ef = (float)((int32_t)kb >> 23);
+    float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
+    return __builtin_fmaf(ef, BQ_LN2_HI_F, __builtin_fmaf(ef, BQ_LN2_LO_F, r));
+}
+
 /* Stirling remainder S(w) = lgamma(w) - (w - 1/2) log w + w - log(2 pi)/2 for
  * w >= 1 in single precision, from iw = 1/w: iw Q(iw^2), Q the degree-6
  * Chebyshev interpolant of w S(w) in u = 1/w^2 on [0, 1]
