@@ -28,12 +28,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-CONFIGS = {
-    "c1": dict(n_b=[300, 300, 200, 200], G=3000, K=5, data_seed=1001, g0=(-2.0, 0.0)),
-    "c2": dict(n_b=[8000, 6000, 4000, 2000], G=3000, K=7, data_seed=1002, g0=(-2.0, 0.0)),
-    "c3": dict(n_b=[40000, 30000, 25000, 20000, 15000, 10000, 6000, 4000], G=5000, K=12,
-               data_seed=1003, g0=(-1.0, 0.5)),
-}
+from busseq_gpu_amd.workloads import CONFIGS, C4_CHAINS, C4_KS, c5_datasets  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 # Every timed sweep is a sweep of the recorded phase of a run (iterations past
 # burn-in and past n_unchanged): the p/tau0 update and the posterior
@@ -97,8 +93,8 @@ def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
     cfg = CONFIGS["c1"]
     n_b, G = cfg["n_b"], cfg["G"]
     Y = bq.simulate(n_b, G, cfg["K"], cfg["data_seed"], gamma0_range=cfg["g0"])
-    Ks = list(range(3, 11))
-    chains = 4
+    Ks = list(C4_KS)
+    chains = C4_CHAINS
     probs = [dict(Y=Y, n_b=n_b, K=K, seed=13579, chains=chains, **PHASE) for K in Ks]
     tr_b, _ = bq.autotune_batch(probs, precision=precision)
     sb = bq.Sampler.batch(probs, precision=precision, tile_rows=tr_b)
@@ -123,23 +119,6 @@ def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
             "sweeps": sweeps, "batched_chain_sweeps_per_s": round(cs_b, 1),
             "plain_sequential_chain_sweeps_per_s": round(cs_p, 1), "ratio": round(cs_b / cs_p, 2),
             "batched_cell_gene_updates_per_s": round(cs_b * N * G, 1)}
-
-
-def c5_datasets(n=16):
-    """c5: 16 synthetic data sets, N log-uniform in [500, 30000], G uniform in
-    [1000, 5000], B in {2..6}, K in {3..10}, batch sizes Dir(2) N with at
-    least K + 1 cells per batch; data seed 5000 + index."""
-    out = []
-    for j in range(n):
-        rng = np.random.default_rng(5000 + j)
-        N = int(round(np.exp(rng.uniform(np.log(500), np.log(30000)))))
-        G = int(rng.integers(1000, 5001))
-        B = int(rng.integers(2, 7))
-        K = int(rng.integers(3, 11))
-        N = max(N, B * (K + 1))
-        extra = rng.multinomial(N - B * (K + 1), rng.dirichlet([2.0] * B))
-        out.append(dict(n_b=[int(K + 1 + e) for e in extra], G=G, K=K, seed=5000 + j))
-    return out
 
 
 def heterogeneous_batch(bq, precision, sweeps=20, warmup=3):

@@ -113,6 +113,7 @@ orc_t* orc_create(int B, const int* n_b, int G, int K, const int32_t* Y,
     }
     size_t NG = (size_t)N * G;
     o->S = (int32_t*)zal(4 * NG);
+    #pragma omp parallel for schedule(static)
     for (size_t e = 0; e < NG; ++e) o->S[e] = Y[e] > 0 ? Y[e] : 0;
     o->W = (int32_t*)zal(4 * N);
     o->L = (int32_t*)zal(4 * (size_t)K * G);
@@ -127,12 +128,23 @@ orc_t* orc_create(int B, const int* n_b, int G, int K, const int32_t* Y,
     o->mu_nu = (double*)zal(8 * B);
     o->mu_delta = (double*)zal(8 * N);
     o->hist_s = (int64_t*)zal(8 * (size_t)B * H);
-    for (int b = 0; b < B; ++b)
-        for (int i = o->boff[b]; i < o->boff[b + 1]; ++i)
-            for (int g = 0; g < G; ++g) {
-                int32_t y = Y[(size_t)i * G + g];
-                if (y > 0 && y < H) o->hist_s[(size_t)b * H + y] += 1;
-            }
+    /* per batch: its cells' rows in parallel, integer counts added (exact in any order) */
+    for (int b = 0; b < B; ++b) {
+        int64_t* hb = o->hist_s + (size_t)b * H;
+        #pragma omp parallel
+        {
+            int64_t* loc = (int64_t*)zal(8 * (size_t)H);
+            #pragma omp for schedule(static)
+            for (int i = o->boff[b]; i < o->boff[b + 1]; ++i)
+                for (int g = 0; g < G; ++g) {
+                    int32_t y = Y[(size_t)i * G + g];
+                    if (y > 0 && y < H) loc[y] += 1;
+                }
+            #pragma omp critical
+            for (int x = 0; x < H; ++x) hb[x] += loc[x];
+            free(loc);
+        }
+    }
     size_t ts = o->prec == 32 ? 4 : 8;
     o->ea = zal(ts * G); o->eb = zal(ts * K * G); o->em = zal(ts * K * G);
     o->bt = zal(ts * K * G); o->en = zal(ts * B * G); o->ed = zal(ts * N);
@@ -226,12 +238,14 @@ int orc_init(orc_t* o)
     const int B = o->B, G = o->G, K = o->K, N = o->N;
     int64_t* lib = (int64_t*)zal(8 * N);
     /* library sizes s_i = sum_g Y_ig (cu:1204-1205) */
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < N; ++i) {
         int64_t s = 0;
         for (int g = 0; g < G; ++g) { int32_t v = o->S[(size_t)i * G + g]; s += v > 0 ? v : 0; }
         lib[i] = s;
-        if (s <= 0) { free(lib); return -1; }
     }
+    for (int i = 0; i < N; ++i)
+        if (lib[i] <= 0) { free(lib); return -1; }
     /* pi0_bk = (k+1) / (K(K+1)/2)  (cu:392-397) */
     for (int b = 0; b < B; ++b)
         for (int k = 0; k < K; ++k) o->pi[b * K + k] = (k + 1) / (K * (K + 1) / 2.0);
@@ -248,15 +262,24 @@ int orc_init(orc_t* o)
     /* raw means r_bkg = mean_{i in b, W_i=k} log(1 + Y_ig / e^delta_i)  (cu:369-382, 1210-1221) */
     uint64_t* rs = (uint64_t*)zal(8 * (size_t)B * K * G);
     int* nbk = (int*)zal(4 * (size_t)B * K);
-    for (int i = 0; i < N; ++i) {
-        int b = o->batch[i], k = o->W[i];
-        nbk[b * K + k] += 1;
-        double ie = bq_exp(-o->delta[i]);
-        uint64_t* row = rs + ((size_t)b * K + k) * G;
-        for (int g = 0; g < G; ++g) {
-            int32_t y = o->S[(size_t)i * G + g];
-            row[g] += bq_fix(bq_log1p((double)(y > 0 ? y : 0) * ie));
+    for (int i = 0; i < N; ++i) nbk[o->batch[i] * K + o->W[i]] += 1;
+    /* exact fixed-point sums: per-thread copies added with integer adds (order-free) */
+    #pragma omp parallel
+    {
+        uint64_t* loc = (uint64_t*)zal(8 * (size_t)B * K * G);
+        #pragma omp for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            int b = o->batch[i], k = o->W[i];
+            double ie = bq_exp(-o->delta[i]);
+            uint64_t* row = loc + ((size_t)b * K + k) * G;
+            for (int g = 0; g < G; ++g) {
+                int32_t y = o->S[(size_t)i * G + g];
+                row[g] += bq_fix(bq_log1p((double)(y > 0 ? y : 0) * ie));
+            }
         }
+        #pragma omp critical
+        for (size_t j = 0; j < (size_t)B * K * G; ++j) rs[j] += loc[j];
+        free(loc);
     }
     double* rm = (double*)zal(8 * (size_t)B * K * G);
     for (int b = 0; b < B; ++b)

@@ -13,14 +13,6 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long CPU test")
 
 
-def gpu_available():
-    try:
-        import torch
-        return torch.cuda.is_available() and torch.cuda.device_count() > 0
-    except Exception:
-        return False
-
-
 @pytest.fixture(scope="session")
 def small_data():
     import busseq_gpu_amd as bq

@@ -10,21 +10,9 @@ import pytest
 
 import busseq_gpu_amd as bq
 from oracle import pyoracle as po
+from parity import compare
 
 pytestmark = pytest.mark.gpu
-
-INT_KEYS = ("X", "Z", "W", "L")
-FLOAT_KEYS = ("alpha", "beta", "nu", "delta", "phi", "gamma", "pi", "ptau")
-
-
-def compare(gs, os_, where):
-    for k in INT_KEYS:
-        a, b = gs[k], os_[k]
-        bad = np.argwhere(a != b)
-        assert bad.size == 0, (f"{where}: {k} differs at {bad[:5].tolist()} "
-                               f"gpu={a[tuple(bad[0])]} oracle={b[tuple(bad[0])]}")
-    for k in FLOAT_KEYS:
-        np.testing.assert_allclose(gs[k], os_[k], rtol=1e-12, atol=1e-300, err_msg=f"{where}: {k}")
 
 
 def run_pair(Y, n_b, K, steps, prec=32, tile_rows=64, **extra):
