@@ -104,10 +104,11 @@ int main(int argc, char** argv)
             std::exit(1);
         }
     }
-    if (B < 0 || B > 200) { std::printf("Error: -B must be between 2 and 128.\n"); return 1; }
+    // the reference's limit (its __constant__ batch arrays, cu:9-10); B < 2 meets its own check below
+    if (B > 128) { std::printf("Error: -B must be at most 128.\n"); return 1; }
     // derived defaults apply only when arguments are given (cu:1092-1111)
     if (argc > 1) {
-        if (n_file_flag) {
+        if (n_file_flag && B > 0) {
             if (busseq_read_dims(n_file, B, n_b.data()) != BUSSEQ_OK) return fail("reading the -N file");
         }
         if (burnin_flag) n_burnin = n_iter / 2;
@@ -119,8 +120,9 @@ int main(int argc, char** argv)
         seed = (int)tv.tv_usec;
     }
     std::printf("Seed is %d\n", seed);
-    // input checks, cu:1120-1134
-    if (n_burnin > n_iter) {
+    // input checks, cu:1120-1134; a burn-in as long as the run is rejected too (the
+    // reference would divide the posterior sums by zero recorded iterations)
+    if (n_burnin >= n_iter) {
         std::printf("Burn-in iterations must be less than total number of iterations.\n");
         return -1;
     }

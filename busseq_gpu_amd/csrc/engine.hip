@@ -1543,9 +1543,11 @@ busseq_status refresh_prep(busseq_sampler* s)
 // launches are a few microseconds apart on the device, consecutive kernels of
 // one graph are not, so long runs replay the longer graph.
 #define BQ_GRAPH_SWEEPS 4
-busseq_status build_graph(busseq_sampler* s)
+void drop_graphs(busseq_sampler* s);
+
+// both graphs, or neither (a failure part way drops what was built)
+busseq_status build_graph_once(busseq_sampler* s)
 {
-    if (s->gexec[0]) return BUSSEQ_OK;
     for (int v = 0; v < 2; ++v) {
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         busseq_status rc = BUSSEQ_OK;
@@ -1559,6 +1561,15 @@ busseq_status build_graph(busseq_sampler* s)
         HIPCHK(hipGraphInstantiate(&s->gexec[v], g, nullptr, nullptr, 0));
     }
     return BUSSEQ_OK;
+}
+
+busseq_status build_graph(busseq_sampler* s)
+{
+    if (s->gexec[0] && s->gexec[1]) return BUSSEQ_OK;
+    drop_graphs(s);
+    busseq_status rc = build_graph_once(s);
+    if (rc != BUSSEQ_OK) drop_graphs(s);
+    return rc;
 }
 
 // n sweeps of every chain, enqueued on the sampler's stream
@@ -1645,7 +1656,9 @@ hipError_t d2h(std::vector<V>& v, const V* src, size_t n)
 busseq_status validate(const busseq_config* cfg)
 {
     if (!cfg) { bq_set_error("null config"); return BUSSEQ_ERR_ARG; }
-    if (cfg->n_burnin > cfg->n_iter) {
+    // equality is rejected too: the run would record nothing, and the reference
+    // (which accepts it, cu:1121) would write posterior means of zero iterations
+    if (cfg->n_burnin >= cfg->n_iter) {
         bq_set_error("Burn-in iterations must be less than total number of iterations.");
         return BUSSEQ_ERR_ARG;
     }
@@ -2315,8 +2328,10 @@ extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result*
     }
     if (c.traces && r->n_pres > 0) {
         // device traces are iteration-major ([iter][param], one coalesced row per
-        // recorded sweep); the result holds them parameter-major ([param][iter])
-        const size_t P = c.n_pres;
+        // recorded sweep); the result holds the recorded ones parameter-major
+        // ([param][iter]) with the stride it reports: after fewer sweeps than
+        // n_iter only the first nrec rows were written
+        const size_t P = (size_t)r->n_pres;
         auto cpd = [&](double*& dst, const double* src, size_t k) -> hipError_t {
             dst = md(k * P);
             std::vector<double> tmp(k * P);

@@ -66,7 +66,19 @@ struct orc {
     int n_rec;
     double *s_gamma, *s_alpha, *s_beta, *s_nu, *s_delta, *s_phi, *s_pi, s_p, s_tau0;
     int64_t* c_L; int32_t* c_W;
+
+    /* traces of the recorded sweeps, the reference's layout [param * n_pres + iter]
+     * (cu:892-936); allocated by orc_keep_traces */
+    int n_pres;
+    double *tr_gamma, *tr_alpha, *tr_beta, *tr_ptau, *tr_nu, *tr_delta, *tr_phi, *tr_pi;
+    int32_t *tr_L, *tr_W;
 };
+
+/* Planted mistakes for the tests that pin the oracle (0 = none): each flips
+ * one rule of the reference to a plausible wrong one, so that a test can show
+ * that its independent check fails on it.  Never set outside tests. */
+static int g_mut = 0;
+void orc_set_mutation(int m) { g_mut = m; }
 
 void orc_set_threads(int n)
 {
@@ -196,7 +208,9 @@ void orc_destroy(orc_t* o)
                    o->beta_sum, o->nu_sum, o->delta_sum, o->phi_sum, o->gamma_sum, o->w_logit,
                    o->acc_alpha, o->acc_beta, o->acc_nu0, o->acc_nu1, o->acc_phi, o->hist_d, o->n1,
                    o->sx1, o->o0, o->o1r, o->o1a, o->cnt, o->perm, o->tiles, o->tl, o->s_gamma, o->s_alpha, o->s_beta,
-                   o->s_nu, o->s_delta, o->s_phi, o->s_pi, o->c_L, o->c_W };
+                   o->s_nu, o->s_delta, o->s_phi, o->s_pi, o->c_L, o->c_W,
+                   o->tr_gamma, o->tr_alpha, o->tr_beta, o->tr_ptau, o->tr_nu, o->tr_delta,
+                   o->tr_phi, o->tr_pi, o->tr_L, o->tr_W };
     for (size_t j = 0; j < sizeof(ps) / sizeof(ps[0]); ++j) free(ps[j]);
     free(o);
 }
@@ -257,7 +271,8 @@ int orc_init(orc_t* o)
         int k = 0; double tmp = o->pi[b * K];
         while (u > tmp && k < K - 1) { k++; tmp += o->pi[b * K + k]; }
         o->W[i] = k;
-        o->delta[i] = bq_log((double)lib[i]) - bq_log((double)lib[o->boff[b]]);
+        int ref = g_mut == 9 ? o->boff[b + 1] - 1 : o->boff[b];   /* planted: last cell of the batch */
+        o->delta[i] = bq_log((double)lib[i]) - bq_log((double)lib[ref]);
     }
     /* raw means r_bkg = mean_{i in b, W_i=k} log(1 + Y_ig / e^delta_i)  (cu:369-382, 1210-1221) */
     uint64_t* rs = (uint64_t*)zal(8 * (size_t)B * K * G);
@@ -286,7 +301,8 @@ int orc_init(orc_t* o)
         for (int k = 0; k < K; ++k)
             for (int g = 0; g < G; ++g) {
                 size_t j = ((size_t)b * K + k) * G + g;
-                rm[j] = nbk[b * K + k] == 0 ? 0.0 : bq_unfix(rs[j]) / nbk[b * K + k];
+                int den = g_mut == 8 ? o->n_b[b] : nbk[b * K + k];   /* planted: batch size */
+                rm[j] = nbk[b * K + k] == 0 ? 0.0 : bq_unfix(rs[j]) / den;
             }
     /* mu_nu_b = mean over batch b of log(1+Y/e^delta) minus batch 0's (cu:470-482, 1232-1241) */
     double* bm = (double*)zal(8 * B);
@@ -297,7 +313,7 @@ int orc_init(orc_t* o)
         bm[b] = (double)(__int128)tot * BQ_FIX_INV / ((double)o->n_b[b] * G);
     }
     o->mu_nu[0] = 0.0;
-    for (int b = 1; b < B; ++b) o->mu_nu[b] = bm[b] - bm[0];
+    for (int b = 1; b < B; ++b) o->mu_nu[b] = g_mut == 3 ? bm[0] - bm[b] : bm[b] - bm[0];   /* 3: planted sign */
     /* gamma0 = (0, -0.3)  (cu:384-390) */
     for (int b = 0; b < B; ++b) { o->gamma[b] = BQ_GAMMA0_INIT; o->gamma[B + b] = BQ_GAMMA1_INIT; }
     /* phi0 = 5, nu0_bg = (sum_k r_bkg - sum_k r_0kg)/K  (cu:399-417) */
@@ -674,7 +690,8 @@ static void update_pi(orc_t* o, int t)
     for (int b = 0; b < B; ++b) {
         bq_rng r; bq_rng_init(&r, o->seed, o->chain, (uint32_t)b, 0u, (uint32_t)t, BQ_PH_PI);
         double gk[16], tot = 0.0;
-        for (int k = 0; k < K; ++k) { gk[k] = bq_rgamma(&r, o->cnt[b * K + k] + BQ_XI_PI, 1.0); tot += gk[k]; }
+        const double xi = g_mut == 1 ? BQ_XI_PI - 1.0 : BQ_XI_PI;   /* 1: planted prior */
+        for (int k = 0; k < K; ++k) { gk[k] = bq_rgamma(&r, o->cnt[b * K + k] + xi, 1.0); tot += gk[k]; }
         for (int k = 0; k < K; ++k) o->pi[b * K + k] = gk[k] / tot;
     }
 }
@@ -744,8 +761,8 @@ static void decide_gamma(orc_t* o, int t)
     }
 }
 
-/* L Gibbs (cu:634-648) then p, tau0 Gibbs (cu:650-668) */
-static void update_L_ptau(orc_t* o, int t)
+/* L Gibbs (cu:634-648) */
+static void update_L(orc_t* o, int t)
 {
     const int G = o->G, K = o->K;
     for (int k = 1; k < K; ++k)
@@ -754,7 +771,12 @@ static void update_L_ptau(orc_t* o, int t)
             double lo = bq_logodds_L(o->p, o->tau0, o->beta[j]);
             o->L[j] = bq_bern_logodds(ph_uniform(o, (uint32_t)g, (uint32_t)k, t, BQ_PH_L), lo);
         }
-    if (t < o->n_unchanged) return;
+}
+
+/* p, tau0 Gibbs (cu:650-668) from the current L and beta */
+static void update_ptau(orc_t* o, int t)
+{
+    const int G = o->G, K = o->K;
     int64_t cnt = 0; uint64_t sb = 0;
     for (int k = 1; k < K; ++k)
         for (int g = 0; g < G; ++g) {
@@ -763,8 +785,9 @@ static void update_L_ptau(orc_t* o, int t)
         }
     double n = (double)G * (K - 1);
     bq_rng r; bq_rng_init(&r, o->seed, o->chain, 0u, 0u, (uint32_t)t, BQ_PH_PTAU);
-    o->p = bq_rbeta(&r, (double)cnt + BQ_A_P, n - (double)cnt + BQ_B_P);
-    double a = BQ_A_TAU0 + (n - (double)cnt) / 2.0;
+    const double b_p = g_mut == 2 ? 1.0 : BQ_B_P;   /* 2: planted prior */
+    o->p = bq_rbeta(&r, (double)cnt + BQ_A_P, n - (double)cnt + b_p);
+    double a = BQ_A_TAU0 + (n - (double)cnt) / (g_mut == 4 ? 1.0 : 2.0);   /* 4: planted shape */
     double bb = BQ_B_TAU0 + bq_unfix(sb) / 2.0;
     o->tau0 = 1.0 / bq_rgamma(&r, a, 1.0 / bb);
 }
@@ -858,6 +881,24 @@ static void record(orc_t* o)
     for (int i = 0; i < N; ++i) { o->s_delta[i] += o->delta[i]; o->c_W[(size_t)i * K + o->W[i]] += 1; }
     for (size_t j = 0; j < (size_t)B * K; ++j) o->s_pi[j] += o->pi[j];
     o->s_p += o->p; o->s_tau0 += o->tau0;
+    const int r = o->n_rec - 1, P = o->n_pres;
+    if (!o->tr_alpha || r >= P) return;
+    for (int j = 0; j < 2 * B; ++j) o->tr_gamma[(size_t)j * P + r] = o->gamma[j];
+    for (int g = 0; g < G; ++g) o->tr_alpha[(size_t)g * P + r] = o->alpha[g];
+    for (size_t j = G; j < (size_t)K * G; ++j) {
+        o->tr_L[(j - G) * P + r] = o->L[j];
+        o->tr_beta[(j - G) * P + r] = o->beta[j];
+    }
+    o->tr_ptau[r] = o->p; o->tr_ptau[P + r] = o->tau0;
+    for (size_t j = 0; j < (size_t)B * G; ++j) {
+        if (j >= (size_t)G) o->tr_nu[(j - G) * P + r] = o->nu[j];
+        o->tr_phi[j * P + r] = o->phi[j];
+    }
+    for (int i = 0; i < N; ++i) {
+        o->tr_delta[(size_t)i * P + r] = o->delta[i];
+        o->tr_W[(size_t)i * P + r] = o->W[i];
+    }
+    for (size_t j = 0; j < (size_t)B * K; ++j) o->tr_pi[j * P + r] = o->pi[j];
 }
 
 static void sweep(orc_t* o)
@@ -870,7 +911,8 @@ static void sweep(orc_t* o)
     if (o->prec == 32) pass_xz_alpha_f32(o, t); else pass_xz_alpha_f64(o, t);
     decide_alpha(o, t);
     decide_gamma(o, t);
-    update_L_ptau(o, t);
+    update_L(o, t);
+    if (t >= o->n_unchanged) update_ptau(o, t);
     propose_beta_nu(o, t);
     if (o->prec == 32) pass_beta_nu_f32(o); else pass_beta_nu_f64(o);
     decide_beta_nu(o, t);
@@ -957,9 +999,12 @@ void orc_posterior(const orc_t* o, orc_post* p)
     for (size_t j = 0; j < (size_t)K * G; ++j) { p->beta[j] = o->s_beta[j] / n; p->L[j] = (double)o->c_L[j] / n; }
     for (size_t j = 0; j < (size_t)B * G; ++j) { p->nu[j] = o->s_nu[j] / n; p->phi[j] = o->s_phi[j] / n; }
     for (int i = 0; i < N; ++i) {
-        p->delta[i] = o->s_delta[i] / n;
+        p->delta[i] = o->s_delta[i] / (g_mut == 10 ? n + 1.0 : n);   /* 10: planted divisor */
         int best = 0, bv = o->c_W[(size_t)i * K];
-        for (int k = 1; k < K; ++k) if (o->c_W[(size_t)i * K + k] > bv) { bv = o->c_W[(size_t)i * K + k]; best = k; }
+        for (int k = 1; k < K; ++k) {
+            const int v = o->c_W[(size_t)i * K + k];
+            if (v > bv || (g_mut == 5 && v == bv)) { bv = v; best = k; }   /* 5: planted tie-break */
+        }
         p->W[i] = best;
     }
     for (size_t j = 0; j < (size_t)B * K; ++j) p->pi[j] = o->s_pi[j] / n;
@@ -993,13 +1038,13 @@ double orc_bic(const orc_t* o, const orc_post* p)
             if (ll[k] > mx) mx = ll[k];
         }
         double s = 0.0;
-        for (int k = 0; k < K; ++k) s += p->pi[b * K + k] * bq_exp(ll[k] - mx);
+        for (int k = 0; k < K; ++k) s += (g_mut == 7 ? 1.0 : p->pi[b * K + k]) * bq_exp(ll[k] - mx);   /* 7: planted */
         lli[i] = mx + bq_log(s);
     }
     double tot = 0.0;
     for (int i = 0; i < N; ++i) tot += lli[i];
     free(lli);
-    double npar = (double)(B + G) * K + 2.0 * B + (double)G * (B * 2 - 1) + N - B;
+    double npar = (double)(B + G) * K + 2.0 * B + (double)G * (B * 2 - 1) + (g_mut == 6 ? 0 : N - B);   /* 6: planted */
     return -2.0 * tot + bq_log((double)G * (double)N) * npar;
 }
 
@@ -1110,4 +1155,48 @@ void orc_terms(int fn, int prec, const double* a, int na, double* out, int n)
         }
         out[j] = res;
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* traces and single steps for the tests                                      */
+/* ------------------------------------------------------------------------ */
+int orc_keep_traces(orc_t* o, int on)
+{
+    if (!on || o->tr_alpha) return 0;
+    const int B = o->B, G = o->G, K = o->K, N = o->N;
+    const size_t P = (size_t)(o->n_iter > o->n_burnin ? o->n_iter - o->n_burnin : 0);
+    o->n_pres = (int)P;
+    o->tr_gamma = (double*)zal(8 * 2 * (size_t)B * P);
+    o->tr_alpha = (double*)zal(8 * (size_t)G * P);
+    o->tr_L = (int32_t*)zal(4 * (size_t)(K - 1) * G * P);
+    o->tr_beta = (double*)zal(8 * (size_t)(K - 1) * G * P);
+    o->tr_ptau = (double*)zal(8 * 2 * P);
+    o->tr_nu = (double*)zal(8 * (size_t)(B - 1) * G * P);
+    o->tr_delta = (double*)zal(8 * (size_t)N * P);
+    o->tr_phi = (double*)zal(8 * (size_t)B * G * P);
+    o->tr_W = (int32_t*)zal(4 * (size_t)N * P);
+    o->tr_pi = (double*)zal(8 * (size_t)B * K * P);
+    return o->tr_alpha && o->tr_W && o->tr_delta ? 0 : -1;
+}
+
+int orc_get_traces(const orc_t* o, orc_traces* t)
+{
+    if (!o->tr_alpha) return 0;
+    const int B = o->B, G = o->G, K = o->K, N = o->N;
+    const size_t P = (size_t)o->n_pres;
+#define CPT(f, n, sz) if (t->f) memcpy(t->f, o->f, (sz) * (size_t)(n) * P)
+    CPT(tr_gamma, 2 * B, 8); CPT(tr_alpha, G, 8); CPT(tr_L, (size_t)(K - 1) * G, 4);
+    CPT(tr_beta, (size_t)(K - 1) * G, 8); CPT(tr_ptau, 2, 8); CPT(tr_nu, (size_t)(B - 1) * G, 8);
+    CPT(tr_delta, N, 8); CPT(tr_phi, (size_t)B * G, 8); CPT(tr_W, N, 4); CPT(tr_pi, (size_t)B * K, 8);
+#undef CPT
+    return o->n_pres;
+}
+
+/* one step of sweep index t on the current state: 0 the pi draws, 1 the p and
+ * tau0 draws, 2 the L draws */
+void orc_step(orc_t* o, int which, int t)
+{
+    if (which == 0) update_pi(o, t);
+    else if (which == 1) update_ptau(o, t);
+    else if (which == 2) update_L(o, t);
 }

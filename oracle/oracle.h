@@ -72,6 +72,26 @@ typedef struct {
 void orc_posterior(const orc_t* o, orc_post* p);
 double orc_bic(const orc_t* o, const orc_post* p);   /* BUSseq_gpu.cu:967-1024,1815-1833 */
 
+/* traces of the recorded sweeps in the reference's layout [param * n_pres + iter]
+ * (BUSseq_gpu.cu:892-936): orc_keep_traces(o, 1) before the first sweep;
+ * orc_get_traces copies them into caller arrays (NULL = skip) and returns n_pres */
+typedef struct {
+    double* tr_gamma; double* tr_alpha; int32_t* tr_L; double* tr_beta; double* tr_ptau;
+    double* tr_nu; double* tr_delta; double* tr_phi; int32_t* tr_W; double* tr_pi;
+} orc_traces;
+int orc_keep_traces(orc_t* o, int on);
+int orc_get_traces(const orc_t* o, orc_traces* t);
+
+/* one step of sweep index t on the current state, for the tests that pin the
+ * conditional draws: 0 pi (cu:856-890), 1 p and tau0 (cu:650-668), 2 L (cu:634-648) */
+void orc_step(orc_t* o, int which, int t);
+
+/* planted mistakes (tests only; 0 = none): 1 pi prior xi = 1, 2 p ~ Beta(., 1),
+ * 3 mu_nu sign, 4 tau0 shape without the /2, 5 W mode ties to the highest k,
+ * 6 BIC penalty without N - B, 7 BIC mixture without pi, 8 raw means over the
+ * batch size, 9 delta0 from the batch's last cell, 10 posterior delta over n + 1 */
+void orc_set_mutation(int m);
+
 /* numerics hooks for the tests (the host build of include/busseq/numerics.h) */
 void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
 void orc_stream(uint32_t seed, uint32_t chain, uint32_t a, uint32_t b, uint32_t sweep,

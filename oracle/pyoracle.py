@@ -50,6 +50,10 @@ def load():
     lib.orc_nb_inv.argtypes = [_P(np.float64)] * 4 + [C.c_int]
     lib.orc_terms.argtypes = [C.c_int, C.c_int, _P(np.float64), C.c_int, _P(np.float64), C.c_int]
     lib.orc_xz_chain.argtypes = [C.c_double] * 4 + [C.c_uint32, C.c_int, C.c_int, _P(np.int64)]
+    lib.orc_keep_traces.argtypes = [vp, C.c_int]
+    lib.orc_get_traces.argtypes = [vp, vp]
+    lib.orc_step.argtypes = [vp, C.c_int, C.c_int]
+    lib.orc_set_mutation.argtypes = [C.c_int]
     _lib = lib
     return lib
 
@@ -109,6 +113,15 @@ def xz_chain(mu, phi, g0, g1, seed, n, nx):
     return out.reshape(nx, 2)
 
 
+TRACE_KEYS = ("gamma", "alpha", "L", "beta", "ptau", "nu", "delta", "phi", "W", "pi")
+STEP = {"pi": 0, "ptau": 1, "L": 2}
+
+
+def set_mutation(m):
+    """Plant mistake m in the oracle (oracle.h lists them; 0 removes it)."""
+    load().orc_set_mutation(int(m))
+
+
 TERMS = {"tmh": 0, "tw": 1, "tphi": 2, "ratio": 3, "lr_gamma1": 4, "lr_phi": 5, "logodds_L": 6,
          "lr_normal": 7, "bic_elem": 8, "lr_beta": 9}
 
@@ -123,7 +136,7 @@ def terms(fn, prec, args):
 
 class Oracle:
     def __init__(self, Y, n_b, K, seed=13579, chain=0, precision=32, n_iter=4000, n_burnin=2000,
-                 n_unchanged=500, tile_rows=64, threads=None):
+                 n_unchanged=500, tile_rows=64, threads=None, keep_traces=False):
         lib = load()
         if threads:
             lib.orc_set_threads(int(threads))
@@ -139,6 +152,9 @@ class Oracle:
         lib.orc_set_tile_rows(h, int(tile_rows))
         if lib.orc_init(h) != 0:
             raise ValueError("oracle init failed (zero library size)")
+        self.n_pres = max(0, n_iter - n_burnin) if keep_traces else 0
+        if keep_traces and lib.orc_keep_traces(h, 1) != 0:
+            raise MemoryError("oracle traces")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -190,6 +206,20 @@ class Oracle:
             else:
                 s[k] = None
         self._lib.orc_set_state(self._h, C.cast(_ptrs([s[k] for k in STATE_KEYS]), C.c_void_p))
+
+    def step(self, which, t):
+        """One step ("pi", "ptau" or "L") of sweep index t on the current state."""
+        self._lib.orc_step(self._h, STEP[which], int(t))
+
+    def traces(self):
+        """Recorded sweeps in the reference's layout: one row per parameter, n_pres columns."""
+        N, G, K, B, P = self.N, self.G, self.K, self.B, self.n_pres
+        t = {"gamma": np.zeros((2 * B, P)), "alpha": np.zeros((G, P)), "L": np.zeros(((K - 1) * G, P), np.int32),
+             "beta": np.zeros(((K - 1) * G, P)), "ptau": np.zeros((2, P)), "nu": np.zeros(((B - 1) * G, P)),
+             "delta": np.zeros((N, P)), "phi": np.zeros((B * G, P)), "W": np.zeros((N, P), np.int32),
+             "pi": np.zeros((B * K, P))}
+        self._lib.orc_get_traces(self._h, C.cast(_ptrs([t[k] for k in TRACE_KEYS]), C.c_void_p))
+        return t
 
     def priors(self):
         ma, mn, md = np.zeros(self.G), np.zeros(self.B), np.zeros(self.N)

@@ -231,6 +231,8 @@ def cli(*args, cwd=None):
 @pytest.mark.parametrize("args,msg", [
     (["-B", "1", "-s", "1", "-i", "10", "-b", "5"], "The batch number must be greater than one."),
     (["-B", "4", "-s", "1", "-i", "10", "-b", "20"], "Burn-in iterations must be less than total number of iterations."),
+    # a burn-in as long as the run records nothing: rejected with the same message
+    (["-B", "4", "-s", "1", "-i", "10", "-b", "10"], "Burn-in iterations must be less than total number of iterations."),
     (["-B", "4", "-K", "301", "-s", "1", "-i", "10", "-b", "5"],
      "The sample size in any batch must be greater than the assumed cell type number."),
 ])
@@ -239,6 +241,19 @@ def test_cli_validation_messages(args, msg):
     assert p.returncode == 255, p.stdout + p.stderr
     assert msg in p.stdout
     assert "Seed is 1" in p.stdout
+
+
+def test_cli_rejects_more_batches_than_the_reference_holds():
+    p = cli("-B", "129", "-s", "1", "-i", "10", "-b", "5")
+    assert p.returncode == 1 and "-B must be at most 128" in p.stdout
+
+
+def test_library_rejects_a_burn_in_as_long_as_the_run():
+    # validation precedes any device call, so this runs without a GPU
+    Y = np.ones((6, 4), np.int32)
+    with pytest.raises(bq.BusseqError) as e:
+        bq.Sampler(Y, [3, 3], 2, n_iter=8, n_burnin=8, n_unchanged=2)
+    assert e.value.status == 1 and "Burn-in iterations must be less" in str(e.value)
 
 
 def test_cli_unknown_option_prints_usage():

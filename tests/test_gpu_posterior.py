@@ -26,6 +26,7 @@ import numpy as np
 import pytest
 
 import busseq_gpu_amd as bq
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +50,29 @@ def test_traces_match_posterior_and_state():
     assert np.array_equal(tr["W"][:, -1], st["W"])
     assert np.array_equal(tr["beta"][:, -1], st["beta"][1:].ravel())
     assert np.array_equal(tr["L"][:, -1], st["L"][1:].ravel())
+
+
+def test_traces_of_a_partial_run_match_the_oracle():
+    # finish after fewer sweeps than n_iter: the result holds the recorded sweeps
+    # only, with the stride it reports (n_pres), and they equal the oracle's traces
+    # (cu:892-936) column by column
+    n_b, G, K = [30, 25, 20], 200, 3
+    Y = bq.simulate(n_b, G, K, 78)
+    kw = dict(seed=6, n_iter=40, n_burnin=15, n_unchanged=5)
+    o = po.Oracle(Y, n_b, K, keep_traces=True, **kw)
+    o.sweeps(27)
+    ot, op = o.traces(), o.posterior()
+    with bq.Sampler(Y, n_b, K, keep_traces=True, **kw) as s:
+        s.sweeps(27)
+        post = s.finish()
+    nrec = 27 - 15
+    assert post["n_pres"] == nrec
+    for key in po.TRACE_KEYS:
+        assert post["traces"][key].shape == (ot[key].shape[0], nrec), key
+        assert np.array_equal(post["traces"][key], ot[key][:, :nrec]), key
+    for key in ("alpha", "delta", "phi", "pi", "gamma", "beta", "nu", "L", "ptau"):
+        np.testing.assert_allclose(post[key], op[key], rtol=1e-12, atol=1e-300, err_msg=key)
+    np.testing.assert_allclose(post["traces"]["alpha"].mean(1), post["alpha"], rtol=1e-12)
 
 
 def mc_se(tr, ess=False):
