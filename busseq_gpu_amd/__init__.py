@@ -61,6 +61,7 @@ class _Result(_C.Structure):
 
 
 _lib = None
+BIC_KEYS = ("alpha", "beta", "nu", "delta", "phi", "gamma", "pi")
 
 
 def _load():
@@ -81,6 +82,7 @@ def _load():
     lib.busseq_get_state.argtypes = [vp, _C.c_int, _C.POINTER(_State)]
     lib.busseq_set_state.argtypes = [vp, _C.c_int, _C.POINTER(_State)]
     lib.busseq_finish.argtypes = [vp, _C.c_int, _C.POINTER(_Result)]
+    lib.busseq_bic.argtypes = [vp, _C.c_int] + [_PD] * 8
     lib.busseq_destroy.argtypes = [vp]
     lib.busseq_destroy.restype = None
     lib.busseq_run.argtypes = [_C.POINTER(_Config), _PI, _C.POINTER(_Result)]
@@ -297,6 +299,14 @@ class Sampler:
                 ("X", "Z", "W", "alpha", "beta", "L", "nu", "delta", "phi", "gamma", "pi", "ptau")}
         st = _State(*[(_ip(v) if k in ("X", "Z", "W", "L") else _dp(v)) for k, v in arrs.items()])
         _check(self._lib.busseq_set_state(self._h, int(chain), _C.byref(st)))
+
+    def bic(self, params, chain=0):
+        """BIC of a chain's counts at parameter values params (a dict with alpha,
+        beta, nu, delta, phi, gamma, pi in the shapes finish() returns)."""
+        arrs = [_np.ascontiguousarray(params[k], _np.float64) for k in BIC_KEYS]
+        out = _C.c_double()
+        _check(self._lib.busseq_bic(self._h, int(chain), *[_dp(a) for a in arrs], _C.byref(out)))
+        return out.value
 
     def finish(self, chain=0):
         r = _Result()

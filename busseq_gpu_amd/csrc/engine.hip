@@ -1424,6 +1424,42 @@ __global__ __launch_bounds__(256) void k_bic(const ChainDev* __restrict__ cds, c
     a = wave_sum_u64(a);
     if (lane == 0) ll[(size_t)i * K + k] = bq_unfix(a);
 }
+
+// Per cell: the subtypes' row sums mixed with the batch's pi (cu:1009-1024),
+// log sum_k pi_bk e^{ll_k}, in place of the cell's first row sum.
+__global__ __launch_bounds__(256) void k_bic_cells(const ChainDev* __restrict__ cds, const double* __restrict__ pi,
+                                                   double* __restrict__ ll)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int K = c.K;
+    if (i >= c.N) return;
+    const int b = c.batch[i];
+    double* li = ll + (size_t)i * K;
+    double mx = -BQ_INF;
+    for (int k = 0; k < K; ++k) if (li[k] > mx) mx = li[k];
+    double sm = 0.0;
+    for (int k = 0; k < K; ++k) sm += pi[(size_t)b * K + k] * bq_exp(li[k] - mx);
+    li[0] = mx + bq_log(sm);
+}
+
+// The cells' values added in cell order (one wave: 64 loaded together, added
+// one by one from the lanes) and the penalty of cu:1833; *out = BIC.
+__global__ __launch_bounds__(64) void k_bic_total(const ChainDev* __restrict__ cds, const double* __restrict__ ll,
+                                                  double* __restrict__ out)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    const int lane = threadIdx.x;
+    const int N = c.N, G = c.G, K = c.K, B = c.B;
+    double tot = 0.0;
+    for (int i0 = 0; i0 < N; i0 += 64) {
+        const double v = i0 + lane < N ? ll[(size_t)(i0 + lane) * K] : 0.0;
+        const int n = N - i0 < 64 ? N - i0 : 64;
+        for (int j = 0; j < n; ++j) tot += read_lane(v, j);
+    }
+    const double npar = (double)(B + G) * K + 2.0 * B + (double)G * (B * 2 - 1) + N - B;
+    if (lane == 0) *out = -2.0 * tot + bq_log((double)G * (double)N) * npar;
+}
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
@@ -2246,6 +2282,79 @@ extern "C" busseq_status busseq_set_state(busseq_sampler* s, int ch, const busse
     return refresh_prep(s);
 }
 
+namespace {
+
+// The BIC's device buffers: the parameters packed as alpha[G] | beta[K*G] |
+// nu[B*G] | delta[N] | phi[B*G] | gamma[2B] | pi[B*K], then N*K row sums, then
+// the result.  Sizes in doubles.
+struct BicLayout {
+    size_t a, be, nu, de, ph, ga, pi, np, ll, out, total;
+    explicit BicLayout(const ChainDev& c)
+    {
+        const size_t G = c.G, K = c.K, B = c.B, N = c.N;
+        a = 0; be = a + G; nu = be + K * G; de = nu + B * G; ph = de + N; ga = ph + B * G;
+        pi = ga + 2 * B; np = pi + B * K;
+        ll = np; out = ll + N * K; total = out + 1;
+    }
+};
+
+// the BIC of chain ch at the given host parameters into buf (bl.total doubles
+// of device memory), result copied to *bic; every word of buf that the kernels
+// read is written first, so buf need not be cleared
+busseq_status bic_into(busseq_sampler* s, int ch, const double* alpha, const double* beta, const double* nu,
+                       const double* delta, const double* phi, const double* gamma, const double* pi,
+                       double* buf, double* bic)
+{
+    const ChainDev& c = s->hd[ch];
+    const BicLayout bl(c);
+    const size_t G = c.G, K = c.K, B = c.B, N = c.N;
+    hipStream_t st = s->stream;
+    HIPCHK(hipMemcpyAsync(buf + bl.a, alpha, 8 * G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(buf + bl.be, beta, 8 * K * G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(buf + bl.nu, nu, 8 * B * G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(buf + bl.de, delta, 8 * N, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(buf + bl.ph, phi, 8 * B * G, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(buf + bl.ga, gamma, 8 * 2 * B, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(buf + bl.pi, pi, 8 * B * K, hipMemcpyHostToDevice, st));
+    const ChainDev* cd = s->d_cd + ch;
+    hipLaunchKernelGGL(k_bic, dim3((unsigned)((N * K + 3) / 4)), dim3(256), 0, st, cd, buf, buf + bl.ll);
+    hipLaunchKernelGGL(k_bic_cells, dim3(nblk((long)N)), dim3(256), 0, st, cd, buf + bl.pi, buf + bl.ll);
+    hipLaunchKernelGGL(k_bic_total, dim3(1), dim3(64), 0, st, cd, buf + bl.ll, buf + bl.out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(bic, buf + bl.out, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return BUSSEQ_OK;
+}
+
+busseq_status bic_args(busseq_sampler* s, int ch, const double* alpha, const double* beta, const double* nu,
+                       const double* delta, const double* phi, const double* gamma, const double* pi, double* bic)
+{
+    if (!s || ch < 0 || ch >= s->C || !alpha || !beta || !nu || !delta || !phi || !gamma || !pi || !bic) {
+        bq_set_error("bad arguments to busseq_bic");
+        return BUSSEQ_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->dev));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return BUSSEQ_OK;
+}
+
+} // namespace
+
+extern "C" busseq_status busseq_bic(busseq_sampler* s, int ch, const double* alpha, const double* beta,
+                                    const double* nu, const double* delta, const double* phi,
+                                    const double* gamma, const double* pi, double* bic)
+{
+    busseq_status rc = bic_args(s, ch, alpha, beta, nu, delta, phi, gamma, pi, bic);
+    if (rc != BUSSEQ_OK) return rc;
+    const BicLayout bl(s->hd[ch]);
+    double* buf = nullptr;
+    HIPCHK(hipMalloc(&buf, bl.total * sizeof(double)));
+    rc = bic_into(s, ch, alpha, beta, nu, delta, phi, gamma, pi, buf, bic);
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(buf);
+    return rc;
+}
+
 extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result* r)
 {
     if (!s || !r || ch < 0 || ch >= s->C) { bq_set_error("bad arguments to busseq_finish"); return BUSSEQ_ERR_ARG; }
@@ -2291,41 +2400,8 @@ extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result*
         r->W[i] = best;
     }
     // BIC at the posterior means (cu:1815-1833)
-    {
-        std::vector<double> pm;
-        pm.insert(pm.end(), r->alpha, r->alpha + G);
-        pm.insert(pm.end(), r->beta, r->beta + (size_t)K * G);
-        pm.insert(pm.end(), r->nu, r->nu + (size_t)B * G);
-        pm.insert(pm.end(), r->delta, r->delta + N);
-        pm.insert(pm.end(), r->phi, r->phi + (size_t)B * G);
-        pm.insert(pm.end(), r->gamma, r->gamma + 2 * (size_t)B);
-        double* d_pm = nullptr;
-        double* d_ll = nullptr;
-        HIPCHK(hipMalloc(&d_pm, pm.size() * 8));
-        HIPCHK(hipMalloc(&d_ll, (size_t)N * K * 8));
-        HIPCHK(hipMemcpy(d_pm, pm.data(), pm.size() * 8, hipMemcpyHostToDevice));
-        const ChainDev* cd = s->d_cd + ch;
-        hipLaunchKernelGGL(k_bic, dim3(((size_t)N * K + 3) / 4), dim3(256), 0, s->stream, cd, d_pm, d_ll);
-        HIPCHK(hipGetLastError());
-        std::vector<double> ll((size_t)N * K);
-        HIPCHK(hipStreamSynchronize(s->stream));
-        HIPCHK(hipMemcpy(ll.data(), d_ll, ll.size() * 8, hipMemcpyDeviceToHost));
-        (void)hipFree(d_pm);
-        (void)hipFree(d_ll);
-        std::vector<int> batch(N);
-        { int i = 0; for (int b = 0; b < B; ++b) for (int q = 0; q < h.n_b[b]; ++q) batch[i++] = b; }
-        double tot = 0.0;
-        for (int i = 0; i < N; ++i) {
-            int b = batch[i];
-            double mx = -BQ_INF;
-            for (int k = 0; k < K; ++k) if (ll[(size_t)i * K + k] > mx) mx = ll[(size_t)i * K + k];
-            double sm = 0.0;
-            for (int k = 0; k < K; ++k) sm += r->pi[(size_t)b * K + k] * bq_exp(ll[(size_t)i * K + k] - mx);
-            tot += mx + bq_log(sm);
-        }
-        double npar = (double)(B + G) * K + 2.0 * B + (double)G * (B * 2 - 1) + N - B;
-        r->bic = -2.0 * tot + bq_log((double)G * (double)N) * npar;
-    }
+    rc = busseq_bic(s, ch, r->alpha, r->beta, r->nu, r->delta, r->phi, r->gamma, r->pi, &r->bic);
+    if (rc != BUSSEQ_OK) return rc;
     if (c.traces && r->n_pres > 0) {
         // device traces are iteration-major ([iter][param], one coalesced row per
         // recorded sweep); the result holds the recorded ones parameter-major

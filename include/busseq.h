@@ -100,6 +100,15 @@ busseq_status busseq_get_state(busseq_sampler* s, int chain, busseq_state* st);
 busseq_status busseq_set_state(busseq_sampler* s, int chain, const busseq_state* st);
 /* posterior + BIC of one chain after the run (BUSseq_gpu.cu:1557-1608, 1815-1833) */
 busseq_status busseq_finish(busseq_sampler* s, int chain, busseq_result* out);
+/* BIC of one chain's counts at the given parameter values (host arrays in the
+ * layouts above: alpha G, beta K*G, nu B*G, delta N, phi B*G, gamma 2*B, pi B*K):
+ * the per-count likelihood with dropout marginalised, the pi-weighted mixture
+ * over subtypes and the penalty of BUSseq_gpu.cu:967-1024, 1815-1833 (which
+ * evaluates it at the posterior means; busseq_finish calls this).  Allocates
+ * (N*K + G*(K+2B+1) + N + 2B + B*K + 1) doubles of device memory per call. */
+busseq_status busseq_bic(busseq_sampler* s, int chain, const double* alpha, const double* beta,
+                         const double* nu, const double* delta, const double* phi, const double* gamma,
+                         const double* pi, double* bic);
 void busseq_destroy(busseq_sampler* s);
 
 /* Batched form (no counterpart in the reference, which runs one chain per

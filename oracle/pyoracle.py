@@ -236,6 +236,16 @@ class Oracle:
         self._lib.orc_get_debug(self._h, C.cast(_ptrs([d[k] for k in DEBUG_KEYS]), C.c_void_p))
         return d
 
+    def bic(self, params):
+        """orc_bic at the given parameters (alpha, beta, nu, delta, phi, gamma, pi)."""
+        N, G, K, B = self.N, self.G, self.K, self.B
+        shapes = {"gamma": (2, B), "alpha": (G,), "L": (K, G), "beta": (K, G), "ptau": (2,), "nu": (B, G),
+                  "delta": (N,), "phi": (B, G), "pi": (B, K)}
+        p = {k: (np.ascontiguousarray(params[k], np.float64).reshape(v) if k in params else np.zeros(v))
+             for k, v in shapes.items()}
+        p["W"] = np.zeros(N, np.int32)
+        return self._lib.orc_bic(self._h, C.cast(_ptrs([p[k] for k in POST_KEYS]), C.c_void_p))
+
     def posterior(self):
         N, G, K, B = self.N, self.G, self.K, self.B
         p = {"gamma": np.zeros((2, B)), "alpha": np.zeros(G), "L": np.zeros((K, G)), "beta": np.zeros((K, G)),

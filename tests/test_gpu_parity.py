@@ -49,6 +49,19 @@ def test_posterior_and_bic_match_oracle(small_data, prec):
     s.close()
 
 
+def test_bic_entry_point_equals_the_oracle_at_any_parameters(small_data):
+    # busseq_bic at the posterior means is finish()'s BIC; at another parameter
+    # set (the chain's current state) it equals the oracle's BIC there
+    d = small_data
+    s, o = run_pair(d["Y"], d["n_b"], d["K"], (30,), 32, n_iter=30, n_burnin=20, n_unchanged=5)
+    r = s.finish()
+    assert s.bic(r) == r["bic"]
+    st = s.state()
+    np.testing.assert_allclose(s.bic(st), o.bic(st), rtol=1e-12)
+    np.testing.assert_allclose(s.bic(o.posterior()), o.posterior()["bic"], rtol=1e-12)
+    s.close()
+
+
 def test_tile_rows_and_k2_b2():
     n_b = [9, 13]
     Y = bq.simulate(n_b, 70, 2, 31)
