@@ -83,6 +83,7 @@ def _load():
     lib.busseq_set_state.argtypes = [vp, _C.c_int, _C.POINTER(_State)]
     lib.busseq_finish.argtypes = [vp, _C.c_int, _C.POINTER(_Result)]
     lib.busseq_bic.argtypes = [vp, _C.c_int] + [_PD] * 8
+    lib.busseq_bic_noalloc.argtypes = [vp, _C.c_int] + [_PD] * 8
     lib.busseq_destroy.argtypes = [vp]
     lib.busseq_destroy.restype = None
     lib.busseq_run.argtypes = [_C.POINTER(_Config), _PI, _C.POINTER(_Result)]
@@ -300,12 +301,14 @@ class Sampler:
         st = _State(*[(_ip(v) if k in ("X", "Z", "W", "L") else _dp(v)) for k, v in arrs.items()])
         _check(self._lib.busseq_set_state(self._h, int(chain), _C.byref(st)))
 
-    def bic(self, params, chain=0):
+    def bic(self, params, chain=0, noalloc=False):
         """BIC of a chain's counts at parameter values params (a dict with alpha,
-        beta, nu, delta, phi, gamma, pi in the shapes finish() returns)."""
+        beta, nu, delta, phi, gamma, pi in the shapes finish() returns); with
+        noalloc, through busseq_bic_noalloc (no device allocation)."""
         arrs = [_np.ascontiguousarray(params[k], _np.float64) for k in BIC_KEYS]
         out = _C.c_double()
-        _check(self._lib.busseq_bic(self._h, int(chain), *[_dp(a) for a in arrs], _C.byref(out)))
+        fn = self._lib.busseq_bic_noalloc if noalloc else self._lib.busseq_bic
+        _check(fn(self._h, int(chain), *[_dp(a) for a in arrs], _C.byref(out)))
         return out.value
 
     def finish(self, chain=0):

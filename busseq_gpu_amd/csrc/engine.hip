@@ -39,6 +39,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -2353,6 +2354,54 @@ extern "C" busseq_status busseq_bic(busseq_sampler* s, int ch, const double* alp
     (void)hipStreamSynchronize(s->stream);
     (void)hipFree(buf);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// busseq_bic_noalloc: the kernels of busseq_bic on a file-scope device array
+// instead of a buffer allocated per call.  Each device holds its own copy of
+// the array from the library's load to the end of the process; its address on
+// a device is taken once, with that device current, and kept per device.
+// ---------------------------------------------------------------------------
+#define BQ_SCRATCH_DOUBLES (4u << 20)   // 32 MiB per device
+#define BQ_SCRATCH_DEVICES 64
+static __device__ __align__(256) double scratch[BQ_SCRATCH_DOUBLES];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[BQ_SCRATCH_DEVICES];
+
+// device address of this device's scratch (the current device is dev)
+static busseq_status scratch_of(int dev, double** out)
+{
+    if (dev < 0 || dev >= BQ_SCRATCH_DEVICES) {
+        bq_set_error("busseq_bic_noalloc: device ordinal beyond " + std::to_string(BQ_SCRATCH_DEVICES - 1));
+        return BUSSEQ_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_bic_noalloc(busseq_sampler* s, int ch, const double* alpha, const double* beta,
+                                            const double* nu, const double* delta, const double* phi,
+                                            const double* gamma, const double* pi, double* bic)
+{
+    busseq_status rc = bic_args(s, ch, alpha, beta, nu, delta, phi, gamma, pi, bic);
+    if (rc != BUSSEQ_OK) return rc;
+    const BicLayout bl(s->hd[ch]);
+    if (bl.total > BQ_SCRATCH_DOUBLES) {
+        bq_set_error("busseq_bic_noalloc: this chain needs " + std::to_string(bl.total) +
+                     " doubles of scratch (N*K + G*(K+2B+1) + N + 2B + B*K + 1), more than its " +
+                     std::to_string(BQ_SCRATCH_DOUBLES) + " (32 MiB); use busseq_bic");
+        return BUSSEQ_ERR_ARG;
+    }
+    double* buf = nullptr;
+    rc = scratch_of(s->dev, &buf);
+    if (rc != BUSSEQ_OK) return rc;
+    return bic_into(s, ch, alpha, beta, nu, delta, phi, gamma, pi, buf, bic);
 }
 
 extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result* r)

@@ -109,6 +109,17 @@ busseq_status busseq_finish(busseq_sampler* s, int chain, busseq_result* out);
 busseq_status busseq_bic(busseq_sampler* s, int chain, const double* alpha, const double* beta,
                          const double* nu, const double* delta, const double* phi, const double* gamma,
                          const double* pi, double* bic);
+/* busseq_bic without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc.  The same kernels and the
+ * same result bit for bit, on a 32 MiB array that each device holds from the
+ * library's load to the end of the process.  Rejects, with BUSSEQ_ERR_ARG, a
+ * chain whose N*K + G*(K+2B+1) + N + 2B + B*K + 1 doubles exceed 4194304.
+ * Calls on one device share that array: calls that overlap in time on one
+ * device (from several threads or samplers) race. */
+busseq_status busseq_bic_noalloc(busseq_sampler* s, int chain, const double* alpha, const double* beta,
+                                 const double* nu, const double* delta, const double* phi,
+                                 const double* gamma, const double* pi, double* bic);
 void busseq_destroy(busseq_sampler* s);
 
 /* Batched form (no counterpart in the reference, which runs one chain per
