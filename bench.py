@@ -15,7 +15,9 @@ Configurations (synthetic counts from the BUSseq generative model):
 The roofline leg times every kernel of the sweep with HIP events on the
 sampler's own stream and reports the dominant one against HBM bandwidth.  The
 cpu_baseline leg times the CPU oracle (OpenMP) on a bounded number of sweeps of
-the same workload on this host.
+the same workload on this host.  Further legs (each skippable): c2 with FP64
+per-element terms, c1 on the GPU beside the CPU oracle, the c4 K-sweep batch
+and the c5 heterogeneous batch against their plain runs in sequence.
 """
 import argparse
 import json
@@ -76,7 +78,7 @@ def cpu_baseline(Y, cfg, precision, budget_s):
     t0 = time.perf_counter()
     o.sweeps(1)
     one = time.perf_counter() - t0
-    n = max(1, min(50, int(budget_s / max(one, 1e-6))))
+    n = max(1, min(2000, int(budget_s / max(one, 1e-6))))
     t0 = time.perf_counter()
     o.sweeps(n)
     dt = time.perf_counter() - t0
@@ -84,6 +86,40 @@ def cpu_baseline(Y, cfg, precision, budget_s):
     return {"value": n / dt, "unit": "Gibbs sweeps/s", "cores": threads, "kind": "port",
             "sample": f"{n} recorded-phase sweeps of the same workload after 1 warm-up sweep, CPU oracle "
                       f"(oracle/oracle.c, OpenMP, {threads} threads, precision {precision})"}
+
+
+def plain_rate(bq, Y, cfg, precision, sweeps, warmup):
+    """Plain form, one chain, tile size autotuned: (sweeps/s, tile rows, kernel ms)."""
+    tr, _ = bq.autotune(Y, cfg["n_b"], cfg["K"], seed=13579, precision=precision, n_burnin=0, n_unchanged=0)
+    with bq.Sampler(Y, cfg["n_b"], cfg["K"], seed=13579, precision=precision, tile_rows=tr, **PHASE) as s:
+        s.sweeps(warmup)
+        ms = s.time_sweeps(sweeps)
+        kt = s.kernel_times(3)
+    return sweeps / (ms / 1e3), tr, kt
+
+
+def c1_leg(bq, precision, cpu_budget, no_cpu, sweeps=400, warmup=20):
+    """c1 (B=4, N=1000, G=3000, K=5): the plain form on the GPU beside the CPU
+    oracle on this host's cores, for the plain form's >= 30x target."""
+    cfg = CONFIGS["c1"]
+    Y = bq.simulate(cfg["n_b"], cfg["G"], cfg["K"], cfg["data_seed"], gamma0_range=cfg["g0"])
+    gpu, tr, _ = plain_rate(bq, Y, cfg, precision, sweeps, warmup)
+    out = {"config": "c1: N=1000 (300/300/200/200), G=3000, K=5, B=4", "sweeps": sweeps, "tile_rows": tr,
+           "gpu_sweeps_per_s": round(gpu, 1), "cpu": None, "gpu_vs_cpu": None, "target": ">= 30x"}
+    if not no_cpu:
+        cpu = cpu_baseline(Y, cfg, precision, cpu_budget)
+        out["cpu"] = cpu
+        out["gpu_vs_cpu"] = round(gpu / cpu["value"], 1)
+    return out
+
+
+def fp64_leg(bq, Y, cfg, sweeps=100, warmup=5):
+    """c2 with every per-element term in double precision (the reference's arithmetic)."""
+    sps, tr, kt = plain_rate(bq, Y, cfg, 64, sweeps, warmup)
+    return {"config": "c2, --precision 64 (per-element NB terms and lgamma in FP64)", "sweeps": sweeps,
+            "tile_rows": tr, "sweeps_per_s": round(sps, 1),
+            "cell_gene_updates_per_s": round(sps * sum(cfg["n_b"]) * cfg["G"], 1),
+            "kernel_ms": {k: round(v, 5) for k, v in kt.items()}}
 
 
 def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
@@ -164,6 +200,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="skip the c1 and FP64 legs")
     ap.add_argument("--tile-rows", type=int, default=0,
                     help="cells per tile of the plain run (0: the autotuner's choice); "
                          "fixed, it keeps separate profiler passes on the same tiles")
@@ -231,6 +268,17 @@ def main():
             f"{hetero['plain_sequential_chain_sweeps_per_s']} -> {hetero['ratio']}x "
             f"({hetero['batched_cell_gene_updates_per_s']:.3g} cell x gene updates/s)")
 
+    fp64 = None
+    if not args.no_legs and args.precision == 32 and args.config == "c2":
+        fp64 = fp64_leg(bq, Y, cfg)
+        log(f"[bench] c2 FP64 terms: {fp64['sweeps_per_s']} sweeps/s (tile rows {fp64['tile_rows']})")
+
+    c1 = None
+    if not args.no_legs and args.config != "c1":
+        c1 = c1_leg(bq, args.precision, args.cpu_budget / 2, args.no_cpu)
+        log(f"[bench] c1: GPU {c1['gpu_sweeps_per_s']} sweeps/s, CPU oracle "
+            f"{c1['cpu']['value'] if c1['cpu'] else float('nan'):.2f} -> {c1['gpu_vs_cpu']}x")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(Y, cfg, args.precision, args.cpu_budget)
@@ -264,6 +312,8 @@ def main():
         "heterogeneous_batch": hetero,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(sweeps_per_s / cpu["value"], 1) if cpu else None,
+        "c1_plain_vs_cpu": c1,
+        "c2_fp64": fp64,
     }
     print(json.dumps(out), flush=True)
 
