@@ -1272,7 +1272,9 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
 // phi sums and k_record use).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 5))) void k_phi(
+// double precision at three waves per SIMD: its lgamma difference keeps more
+// values live than five (or four) waves' registers hold without spills
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 3))) void k_phi(
     const ChainDev* __restrict__ cds, int gy)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];

@@ -114,6 +114,17 @@ def main():
 
     co, w = report("STIRLING_Q_F32", stirling_q, 0, 1, 6, f32, abs_s)
     out.append(("STIRLING_Q_F32", co, w))
+
+    # double precision, 1 <= w <= 10: w S(w) in t = 1/w on [0.1, 1] (slightly
+    # widened); w >= 10 keeps the Stirling series.  The error is absolute, in S = t Q(t)
+    def stirling_t(t):
+        return stirling_q(t * t)
+
+    def abs_st(t, q):
+        return (q - stirling_t(t)) * t
+
+    co, w = report("STIRLING_T_F64", stirling_t, mp.mpf("0.0999"), mp.mpf("1.0001"), 20, f64, abs_st)
+    out.append(("STIRLING_T_F64", co, w))
     for name, co, w in out:
         print(f"/* {name}: degree {len(co)-1}, max err of the rounded polynomial {mp.nstr(w, 3)} */")
         print(f"#define {name} \\")

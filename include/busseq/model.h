@@ -106,8 +106,13 @@ BQ_FN T bq_tphi_gen_##SFX(int xi, T x, T phi, T phip, T mu, T d, T dop, T iphip,
 BQ_DEFINE_ELEMENT_TERMS(float, f32, bq_log1pf, bq_lgamma_diff32, __builtin_fmaf)
 BQ_DEFINE_ELEMENT_TERMS(double, f64, bq_log1p, bq_lgamma_diff64, __builtin_fma)
 
+/* The double-precision subtype logit and phi terms take the branch-free
+ * bq_log1p_mid: their arguments (r0 expm1(beta) > -1; d mu / (phi (phi' + mu))
+ * > -1; mu / phi' > 0) often leave bq_log1p's fast range, where a branch would
+ * run both paths for the wave; the random-walk terms' arguments r c (|c| about
+ * 0.1) stay inside it and keep bq_log1p. */
 BQ_FN double bq_tw_f64(double x, double phi, double r0, double beta, double em)
-{ return bq_tw_gen_f64(x, phi, r0, beta, em); }
+{ return __builtin_fma(-(phi + x), bq_log1p_mid(r0 * em), x * beta); }
 
 /* The single-precision subtype logit term: expm1(beta) is held to
  * [-1 + 2^-24, 2^100] (beta beyond about +-69 only), so r0 expm1(beta) with
@@ -120,7 +125,13 @@ BQ_FN float bq_tw_f32(float x, float phi, float r0, float beta, float em)
 
 BQ_FN double bq_tphi_f64(int xi, double x, double phi, double phip, double mu, double d, double dop,
                          double iphip, double lrat, double dlg0)
-{ return bq_tphi_gen_f64(xi, x, phi, phip, mu, d, dop, iphip, lrat, dlg0); }
+{
+    double l1 = bq_log1p_mid((dop * mu) / (phip + mu));
+    double l2 = bq_log1p_mid(mu * iphip);
+    double t = __builtin_fma(-d, l2, __builtin_fma(phi + x, l1, -(x * lrat)));
+    if (xi > 0) t = t + (bq_lgamma_diff64(phi + x, d) - dlg0);
+    return t;
+}
 
 /* The single-precision phi term: the expression of bq_tphi_gen_f32 with its two
  * reciprocals, 1/(phi' + mu) and the Stirling difference's 1/((phi + x)(phi' + x)),

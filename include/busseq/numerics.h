@@ -207,6 +207,22 @@ BQ_FN double bq_log1p(double y)
     return __builtin_fma(ed, BQ_LN2_HI, __builtin_fma(ed, BQ_LN2_LO, r));
 }
 
+/* log1p(y) for finite y > -1 without the special-value and range branches of
+ * bq_log1p, for per-element terms whose arguments are known to lie there:
+ * 1 + y = 2^k m with m in [sqrt(1/2), sqrt(2)), k from the rounded sum t = 1 + y,
+ * and the reduced argument f = (1 + y) 2^-k - 1 = fma(y, 2^-k, 2^-k - 1) formed
+ * from the unrounded y with one rounding (f = y exactly for k = 0) */
+BQ_FN double bq_log1p_mid(double y)
+{
+    double t = 1.0 + y;
+    uint64_t kb = (bq_dbits(t) - 0x3FE6A09E667F3BCDull) & 0xFFF0000000000000ull;   /* k << 52 */
+    double s = bq_bitsd(0x3FF0000000000000ull - kb);                                  /* 2^-k */
+    double f = __builtin_fma(y, s, s - 1.0);
+    double ed = (double)((int64_t)kb >> 52);
+    double r = __builtin_fma(f * f, bq_log1p_q64(f), f);
+    return __builtin_fma(ed, BQ_LN2_HI, __builtin_fma(ed, BQ_LN2_LO, r));
+}
+
 /* (exp(r) - 1 - r) / r^2 on |r| <= ln2/2, degree 11, rel err 6e-19 */
 BQ_FN double bq_exp_e64(double r)
 {
@@ -261,7 +277,7 @@ BQ_FN double bq_softplus(double t)
     return bq_log1p(bq_exp(t));
 }
 
-/* Stirling remainder sum_{n>=1} B_2n / (2n(2n-1) w^(2n-1)) for w >= 10 */
+/* Stirling remainder sum_{n>=1} B_2n / (2n(2n-1) w^(2n-1)) for w >= 10 (8 terms, error < 3e-17) */
 BQ_FN double bq_stirling_s64(double iw)
 {
     double z = iw * iw;
@@ -276,7 +292,43 @@ BQ_FN double bq_stirling_s64(double iw)
     return p * iw;
 }
 
-/* log Gamma(z) for z > 0 */
+/* w S(w) for 1 <= w <= 10 from t = 1/w: the degree-20 Chebyshev interpolant on
+ * t in [0.1, 1] (busseq_gpu_amd/csrc/gen_coeffs.py, STIRLING_T_F64; abs err of
+ * S = t Q(t) below 1.1e-17 with the rounded coefficients) */
+BQ_FN double bq_stirling_q64_t(double t)
+{
+    double p = -3.43114355651204472287e-5;
+    p = __builtin_fma(p, t, 4.2004461293514551521e-4);
+    p = __builtin_fma(p, t, -2.41949115872181234541e-3);
+    p = __builtin_fma(p, t, 8.71150149019403637063e-3);
+    p = __builtin_fma(p, t, -2.19576369828119316585e-2);
+    p = __builtin_fma(p, t, 4.10800329988699128303e-2);
+    p = __builtin_fma(p, t, -5.89271741857547368126e-2);
+    p = __builtin_fma(p, t, 6.58297525301950320342e-2);
+    p = __builtin_fma(p, t, -5.72932315726428567637e-2);
+    p = __builtin_fma(p, t, 3.80745376197201002011e-2);
+    p = __builtin_fma(p, t, -1.82074013536179177275e-2);
+    p = __builtin_fma(p, t, 5.30895500113091195482e-3);
+    p = __builtin_fma(p, t, -5.12545438014087558279e-4);
+    p = __builtin_fma(p, t, 2.72928504362822963239e-4);
+    p = __builtin_fma(p, t, -6.38592612214043784009e-4);
+    p = __builtin_fma(p, t, 5.36487805755688477614e-6);
+    p = __builtin_fma(p, t, 7.93144926255619697394e-4);
+    p = __builtin_fma(p, t, 3.50466592371063494979e-8);
+    p = __builtin_fma(p, t, -2.77777945586209912276e-3);
+    p = __builtin_fma(p, t, 4.94941088224170359986e-11);
+    p = __builtin_fma(p, t, 8.33333333326573694189e-2);
+    return p;
+}
+
+/* Stirling remainder S(w) for w >= 1 from iw = 1/w: the fit below 10, the series above */
+BQ_FN double bq_stirling_s64_w1(double w, double iw)
+{
+    return w >= 10.0 ? bq_stirling_s64(iw) : iw * bq_stirling_q64_t(iw);
+}
+
+/* log Gamma(z) for z > 0 (priors, the Poisson sampler, the BIC: scalar or rare
+ * per element, so the shift to 10 and the series serve) */
 BQ_FN double bq_lgamma(double z)
 {
     if (!(z > 0.0)) return z == 0.0 ? BQ_INF : BQ_NAN;
@@ -435,15 +487,19 @@ BQ_FN float bq_lgamma_diff32(float z, float d)
     return dl;
 }
 
+/* the double-precision form: the same Stirling difference with S from
+ * bq_stirling_s64_w1, valid from 1, so the phi term's arguments phi + x
+ * (x >= 1) are never shifted (a shift to 10 ran up to nine steps of a loop and
+ * two more logarithms on most lanes) */
 BQ_FN double bq_lgamma_diff64(double z, double d)
 {
     double zp = z + d;
     double pz = 1.0, pzp = 1.0;
     int shifted = 0;
-    while (z < 10.0 || zp < 10.0) { pz *= z; pzp *= zp; z += 1.0; zp += 1.0; shifted = 1; }
+    while (z < 1.0 || zp < 1.0) { pz *= z; pzp *= zp; z += 1.0; zp += 1.0; shifted = 1; }
     double iz = 1.0 / z, izp = 1.0 / zp;
-    double core = __builtin_fma(d, bq_log(zp), __builtin_fma(z - 0.5, bq_log1p(d * iz), -d));
-    double dl = core + (bq_stirling_s64(izp) - bq_stirling_s64(iz));
+    double core = __builtin_fma(d, bq_log(zp), __builtin_fma(z - 0.5, bq_log1p_mid(d * iz), -d));
+    double dl = core + (bq_stirling_s64_w1(zp, izp) - bq_stirling_s64_w1(z, iz));
     if (shifted) dl = dl - (bq_log(pzp) - bq_log(pz));
     return dl;
 }

@@ -112,6 +112,47 @@ def test_lgamma_diff32_on_the_phi_term_domain():
     assert (np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)).max() < 5e-7
 
 
+def test_lgamma_diff64_on_the_phi_term_domain():
+    # the double-precision phi term evaluates lgamma(phi' + x) - lgamma(phi + x) with
+    # x >= 1: arguments above 1, where the fitted Stirling remainder (1 <= w < 10) and
+    # the series (w >= 10) are used without a shift; 40-digit mpmath
+    import mpmath as mp
+    rng = np.random.default_rng(12)
+    n = 2000
+    phi = np.exp(rng.uniform(np.log(0.02), np.log(60), n))
+    x = np.where(rng.random(n) < 0.5, rng.integers(1, 2001, n), rng.integers(1, 12, n)).astype(np.float64)
+    z = phi + x
+    d = rng.gamma(phi, 1.0) - phi
+    got = po.math("lgamma_diff64", z, d)
+    with mp.workdps(40):
+        ref = np.array([float(mp.loggamma(mp.mpf(a) + mp.mpf(b)) - mp.loggamma(mp.mpf(a))) for a, b in zip(z, d)])
+    assert (np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)).max() < 2e-15
+
+
+def test_lgamma_diff64_across_the_fit_and_series_ranges():
+    # the Stirling remainder switches from the fit to the series at w = 10: arguments
+    # on both sides of it and at it, against 40-digit mpmath
+    import mpmath as mp
+    z = np.concatenate([np.linspace(1.0, 12.0, 1101), [np.nextafter(10.0, 0), 10.0, np.nextafter(10.0, 20)]])
+    d = np.concatenate([np.full(1101, 0.75), [0.5, -0.5, 1e-9]])
+    with mp.workdps(40):
+        ref = np.array([float(mp.loggamma(mp.mpf(float(a)) + mp.mpf(float(b))) - mp.loggamma(mp.mpf(float(a))))
+                        for a, b in zip(z, d)])
+    err = np.abs(po.math("lgamma_diff64", z, d) - ref) / np.maximum(np.abs(ref), 1.0)
+    assert err.max() < 2e-15
+
+
+def test_log1p_mid_vs_mpmath():
+    # the branch-free log1p of the double-precision element terms, on (-1, 1e12)
+    import mpmath as mp
+    rng = np.random.default_rng(5)
+    y = np.concatenate([rng.uniform(-0.999999, 3, 1500), rng.uniform(-1e-8, 1e-8, 200), -1 + np.exp(rng.uniform(-30, -1, 200)),
+                        np.exp(rng.uniform(-40, 27, 400)), [0.0, 0.41421356237309515, -0.29289321881345248]])
+    with mp.workdps(40):
+        ref = np.array([float(mp.log1p(mp.mpf(float(v)))) for v in y])
+    assert ulps(po.math("log1p_mid", y), ref).max() <= 2.0
+
+
 def test_single_precision_logs_vs_mpmath(gold):
     x = gold["log_x"].astype(np.float32)
     x = x[np.isfinite(x) & (x > 0)]
