@@ -26,6 +26,7 @@
     int N, G, Gp, K, B; \
     uint32_t seed, key;  /* Philox key words: (seed, chain id) */ \
     int n_burnin, n_unchanged, n_pres, traces; \
+    int tr_rows;  /* rows of the trace ring: sweep t is written to row t mod tr_rows */ \
     int TR, T_max;  /* rows per tile, tile capacity */ \
     int prec;  /* 32 or 64 */ \
     /* static data */ \
@@ -77,7 +78,8 @@
     Q int* iter_rec;  /* [1] the sweep index k_record uses, copied from iter by k_bn_dec */ \
     Q double *s_gamma, *s_alpha, *s_beta, *s_nu, *s_delta, *s_phi, *s_pi, *s_ptau; \
     Q int32_t *c_L, *c_W; \
-    /* traces, iteration-major: [r * n_params + param] */ \
+    /* traces: a ring of tr_rows recorded sweeps, iteration-major ([row * n_params + param]), \
+     * streamed to host memory in blocks while the chain runs */ \
     Q double *tr_gamma, *tr_alpha, *tr_beta, *tr_ptau, *tr_nu, *tr_delta, *tr_phi, *tr_pi; \
     Q int32_t *tr_L, *tr_W; \
     /* end */
@@ -103,6 +105,11 @@ struct HostInit {
 void bq_init_cells(int B, const int* n_b, int K, const int64_t* lib, uint32_t seed, uint32_t chain, HostInit& o);
 void bq_init_params(int B, const int* n_b, int G, int K, const uint64_t* rs, uint32_t seed, uint32_t chain,
                     HostInit& o);
+
+// host.cpp: rows of a streamed trace block (rows x np, row-major) into the
+// parameter-major host traces dst (np x P): row q to column col[q], skipped
+// where col[q] < 0; esz is 4 or 8 bytes
+void bq_trace_transpose(const void* stage, int esz, size_t np, int rows, const int* col, size_t P, void* dst);
 
 // init.hip: the passes over the counts (hipError_t values as int)
 // rows: S (Y > 0 -> Y, else 0, padding 0), per-row counts of zero entries and

@@ -1310,8 +1310,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void record_sweep(const ChainG& c, int t, int j, int G, int K, int B, int N, int P)
 {
-    const int r = t - c.n_burnin;
-    const bool tr = c.traces && r < P;
+    // recorded sweep r = t - n_burnin (t >= n_burnin here) goes to ring row t mod tr_rows
+    const bool tr = c.traces && t - c.n_burnin < P;
+    const int r = tr ? t % c.tr_rows : 0;
     if (j == 0) *c.n_rec += 1;
     if (j < 2 * B) {
         c.s_gamma[j] += c.gamma[j];
@@ -1483,6 +1484,22 @@ struct ChainHost {
     int T_alloc;                   // tiles the partial-sum buffer holds
 };
 
+// Streamed traces (-p): each chain's device ring holds 2 blocks of
+// BQ_TRACE_BLOCK sweeps.  When the sweeps of block k are enqueued, its rows are
+// copied (on the copy stream, after them) into pinned host memory, and the
+// previous block, whose copy has finished by then, is transposed into the
+// host traces while the GPU runs block k.
+#define BQ_TRACE_BLOCK 32      // sweeps per streamed block (a multiple of the 4-sweep graph)
+#define BQ_TRACE_ARRAYS 10     // gamma alpha L beta ptau nu delta phi W pi
+struct TraceHost {
+    int P = 0, nb = 0;                       // recorded sweeps kept, burn-in
+    size_t np[BQ_TRACE_ARRAYS] = {};         // parameters per row
+    int esz[BQ_TRACE_ARRAYS] = {};           // bytes per value
+    void* dev[BQ_TRACE_ARRAYS] = {};         // device ring (2 blocks of rows)
+    void* stage[BQ_TRACE_ARRAYS] = {};       // pinned copy of the ring
+    void* host[BQ_TRACE_ARRAYS] = {};        // [param * P + iter]
+};
+
 struct busseq_sampler {
     int C = 0;                     // chains
     int prec = 32, TR = 64, dev = 0;
@@ -1496,8 +1513,15 @@ struct busseq_sampler {
     hipGraph_t graph[2] = {};      // one sweep, BQ_GRAPH_SWEEPS sweeps
     hipGraphExec_t gexec[2] = {};
     std::vector<void*> allocs;
-    int iter = 0;
+    int iter = 0;                  // sweeps enqueued
     bool oom = false;              // some dalloc failed
+    // trace streaming
+    std::vector<TraceHost> th;     // per chain (P = 0: no traces)
+    bool tracing = false;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_sweeps = nullptr, ev_copy[2] = {};
+    bool copy_recorded[2] = {};    // ev_copy[h] has been recorded (streams wait only on recorded events)
+    int pend_block = -1, pend_rows = 0;   // block whose copy is in flight, not yet transposed
 
     template <typename V>
     V* dalloc(size_t n)
@@ -1611,11 +1635,135 @@ busseq_status build_graph(busseq_sampler* s)
     return rc;
 }
 
-// n sweeps of every chain, enqueued on the sampler's stream
+// ---------------------------------------------------------------------------
+// trace streaming (see TraceHost)
+// ---------------------------------------------------------------------------
+// The first `rows` rows of block k (ring half k & 1) of every traced chain into
+// its pinned copy, on the copy stream after the sweeps enqueued so far.
+busseq_status trace_copy(busseq_sampler* s, int k, int rows)
+{
+    const int half = k & 1;
+    HIPCHK(hipEventRecord(s->ev_sweeps, s->stream));
+    HIPCHK(hipStreamWaitEvent(s->copy_stream, s->ev_sweeps, 0));
+    for (int c = 0; c < s->C; ++c) {
+        const TraceHost& t = s->th[c];
+        if (!t.P) continue;
+        for (int a = 0; a < BQ_TRACE_ARRAYS; ++a) {
+            const size_t row = t.np[a] * (size_t)t.esz[a];
+            if (!row) continue;
+            const size_t off = (size_t)half * BQ_TRACE_BLOCK * row;
+            HIPCHK(hipMemcpyAsync((char*)t.stage[a] + off, (const char*)t.dev[a] + off, (size_t)rows * row,
+                                  hipMemcpyDeviceToHost, s->copy_stream));
+        }
+    }
+    HIPCHK(hipEventRecord(s->ev_copy[half], s->copy_stream));
+    s->copy_recorded[half] = true;
+    s->pend_block = k;
+    s->pend_rows = rows;
+    return BUSSEQ_OK;
+}
+
+// The block whose copy is in flight, once it has arrived, into every traced
+// chain's host traces (rows of sweeps before burn-in or past n_pres skipped).
+busseq_status trace_drain(busseq_sampler* s)
+{
+    if (s->pend_block < 0) return BUSSEQ_OK;
+    const int k = s->pend_block, half = k & 1, rows = s->pend_rows;
+    s->pend_block = -1;
+    HIPCHK(hipEventSynchronize(s->ev_copy[half]));
+    int col[BQ_TRACE_BLOCK];
+    for (int c = 0; c < s->C; ++c) {
+        TraceHost& t = s->th[c];
+        if (!t.P) continue;
+        bool any = false;
+        for (int q = 0; q < rows; ++q) {
+            const int r = k * BQ_TRACE_BLOCK + q - t.nb;
+            col[q] = r >= 0 && r < t.P ? r : -1;
+            any = any || col[q] >= 0;
+        }
+        if (!any) continue;
+        for (int a = 0; a < BQ_TRACE_ARRAYS; ++a) {
+            if (!t.np[a]) continue;
+            if (!t.host[a]) {
+                t.host[a] = std::calloc(t.np[a] * (size_t)t.P, (size_t)t.esz[a]);
+                if (!t.host[a]) { bq_set_error("host allocation of traces failed"); return BUSSEQ_ERR_NOMEM; }
+            }
+            const size_t row = t.np[a] * (size_t)t.esz[a];
+            bq_trace_transpose((const char*)t.stage[a] + (size_t)half * BQ_TRACE_BLOCK * row, t.esz[a], t.np[a],
+                               rows, col, (size_t)t.P, t.host[a]);
+        }
+    }
+    return BUSSEQ_OK;
+}
+
+// whether some chain records a sweep of block k
+bool trace_block_used(const busseq_sampler* s, int k)
+{
+    const int t0 = k * BQ_TRACE_BLOCK, t1 = t0 + BQ_TRACE_BLOCK;
+    for (const TraceHost& t : s->th)
+        if (t.P && t1 > t.nb && t0 < t.nb + t.P) return true;
+    return false;
+}
+
+// After the sweeps of block k are enqueued: the previous block (its copy ran
+// behind its sweeps) into the host traces, this block's copy enqueued, and the
+// next block's sweeps held until the copy out of the ring half they overwrite
+// has finished.
+busseq_status trace_boundary(busseq_sampler* s, int k)
+{
+    if (trace_block_used(s, k)) {
+        busseq_status rc = trace_drain(s);
+        if (rc != BUSSEQ_OK) return rc;
+        rc = trace_copy(s, k, BQ_TRACE_BLOCK);
+        if (rc != BUSSEQ_OK) return rc;
+    }
+    if (s->copy_recorded[(k + 1) & 1]) HIPCHK(hipStreamWaitEvent(s->stream, s->ev_copy[(k + 1) & 1], 0));
+    return BUSSEQ_OK;
+}
+
+// every recorded sweep so far into the host traces (the sweeps since the last
+// block boundary too)
+busseq_status trace_flush(busseq_sampler* s)
+{
+    if (!s->tracing) return BUSSEQ_OK;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    busseq_status rc = trace_drain(s);
+    if (rc != BUSSEQ_OK) return rc;
+    const int k = s->iter / BQ_TRACE_BLOCK, rows = s->iter % BQ_TRACE_BLOCK;
+    if (rows > 0 && trace_block_used(s, k)) {
+        rc = trace_copy(s, k, rows);
+        if (rc == BUSSEQ_OK) rc = trace_drain(s);
+    }
+    return rc;
+}
+
+// one sweep enqueued outside the graphs (busseq_kernel_times)
+busseq_status count_sweep(busseq_sampler* s)
+{
+    s->iter += 1;
+    if (s->tracing && s->iter % BQ_TRACE_BLOCK == 0) return trace_boundary(s, s->iter / BQ_TRACE_BLOCK - 1);
+    return BUSSEQ_OK;
+}
+
+// n sweeps of every chain, enqueued on the sampler's stream; with traces, in
+// runs that end at the streamed blocks' boundaries
 busseq_status launch_sweeps(busseq_sampler* s, int n)
 {
-    for (; n >= BQ_GRAPH_SWEEPS; n -= BQ_GRAPH_SWEEPS) HIPCHK(hipGraphLaunch(s->gexec[1], s->stream));
-    for (; n > 0; --n) HIPCHK(hipGraphLaunch(s->gexec[0], s->stream));
+    while (n > 0) {
+        int m = n;
+        if (s->tracing) {
+            const int left = BQ_TRACE_BLOCK - s->iter % BQ_TRACE_BLOCK;
+            if (left < m) m = left;
+        }
+        n -= m;
+        s->iter += m;
+        for (; m >= BQ_GRAPH_SWEEPS; m -= BQ_GRAPH_SWEEPS) HIPCHK(hipGraphLaunch(s->gexec[1], s->stream));
+        for (; m > 0; --m) HIPCHK(hipGraphLaunch(s->gexec[0], s->stream));
+        if (s->tracing && s->iter % BQ_TRACE_BLOCK == 0) {
+            busseq_status rc = trace_boundary(s, s->iter / BQ_TRACE_BLOCK - 1);
+            if (rc != BUSSEQ_OK) return rc;
+        }
+    }
     return BUSSEQ_OK;
 }
 
@@ -1887,7 +2035,8 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.N = N; c.G = G; c.Gp = Gp; c.K = K; c.B = B;
             c.seed = cfg->seed; c.key = key;
             c.n_burnin = cfg->n_burnin; c.n_unchanged = cfg->n_unchanged;
-            c.n_pres = P; c.traces = cfg->keep_traces ? 1 : 0;
+            c.n_pres = P; c.traces = cfg->keep_traces && P > 0 ? 1 : 0;
+            c.tr_rows = c.traces ? 2 * BQ_TRACE_BLOCK : 1;
             c.TR = TR; c.T_max = T_max; c.prec = s->prec;
             c.batch = pd.batch; c.boff = pd.boff; c.hist_s = pd.hist;
             c.zrow = pd.zrow; c.zcol = pd.zcol; c.big_off = pd.big_off; c.big = pd.big;
@@ -1935,14 +2084,33 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.s_delta = s->dalloc<double>(N); c.s_phi = s->dalloc<double>((size_t)B * G);
             c.s_pi = s->dalloc<double>((size_t)B * K); c.s_ptau = s->dalloc<double>(2);
             c.c_L = s->dalloc<int32_t>((size_t)K * G); c.c_W = s->dalloc<int32_t>((size_t)N * K);
-            if (c.traces && P > 0) {
-                c.tr_gamma = s->dalloc<double>(2 * (size_t)B * P); c.tr_alpha = s->dalloc<double>((size_t)G * P);
-                c.tr_beta = s->dalloc<double>((size_t)(K - 1) * G * P); c.tr_ptau = s->dalloc<double>(2 * (size_t)P);
-                c.tr_nu = s->dalloc<double>((size_t)(B - 1) * G * P); c.tr_delta = s->dalloc<double>((size_t)N * P);
-                c.tr_phi = s->dalloc<double>((size_t)B * G * P); c.tr_pi = s->dalloc<double>((size_t)B * K * P);
-                c.tr_L = s->dalloc<int32_t>((size_t)(K - 1) * G * P); c.tr_W = s->dalloc<int32_t>((size_t)N * P);
-                if (!c.tr_W || !c.tr_L || !c.tr_delta) { bq_set_error("device allocation of traces failed"); return fail(BUSSEQ_ERR_NOMEM); }
+            TraceHost th;
+            if (c.traces) {
+                // rings of 2 streamed blocks of rows; their pinned copies
+                const size_t R2 = (size_t)c.tr_rows;
+                const size_t np[BQ_TRACE_ARRAYS] = {2 * (size_t)B, (size_t)G, (size_t)(K - 1) * G, (size_t)(K - 1) * G, 2,
+                                                    (size_t)(B - 1) * G, (size_t)N, (size_t)B * G, (size_t)N, (size_t)B * K};
+                const int esz[BQ_TRACE_ARRAYS] = {8, 8, 4, 8, 8, 8, 8, 8, 4, 8};
+                void** dev[BQ_TRACE_ARRAYS] = {(void**)&c.tr_gamma, (void**)&c.tr_alpha, (void**)&c.tr_L, (void**)&c.tr_beta,
+                                               (void**)&c.tr_ptau, (void**)&c.tr_nu, (void**)&c.tr_delta, (void**)&c.tr_phi,
+                                               (void**)&c.tr_W, (void**)&c.tr_pi};
+                th.P = P;
+                th.nb = cfg->n_burnin;
+                for (int a = 0; a < BQ_TRACE_ARRAYS; ++a) {
+                    th.np[a] = np[a];
+                    th.esz[a] = esz[a];
+                    *dev[a] = s->dalloc<char>(R2 * np[a] * esz[a]);
+                    th.dev[a] = *dev[a];
+                    if (hipHostMalloc(&th.stage[a], R2 * np[a] * esz[a] + 1, hipHostMallocDefault) != hipSuccess) {
+                        th.stage[a] = nullptr;
+                        s->th.push_back(th);
+                        bq_set_error("pinned host allocation of the trace blocks failed");
+                        return fail(BUSSEQ_ERR_NOMEM);
+                    }
+                }
+                s->tracing = true;
             }
+            s->th.push_back(th);
             if (s->oom) {
                 bq_set_error("device allocation failed");
                 return fail(BUSSEQ_ERR_NOMEM);
@@ -1986,6 +2154,15 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
         if (!s->krange.empty() && s->ch[s->krange.back().first].K == s->ch[c].K) s->krange.back().second += 1;
         else s->krange.push_back({c, 1});
     }
+    if (s->tracing) {
+        if (hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_sweeps, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_copy[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_copy[1], hipEventDisableTiming) != hipSuccess) {
+            bq_set_error("creating the trace copy stream failed");
+            return fail(BUSSEQ_ERR_DEVICE);
+        }
+    }
     if (hipMalloc(&s->d_cd, sizeof(ChainDev) * s->C) != hipSuccess) { bq_set_error("device allocation failed"); return fail(BUSSEQ_ERR_NOMEM); }
     if (hipMemcpy(s->d_cd, s->hd.data(), sizeof(ChainDev) * s->C, hipMemcpyHostToDevice) != hipSuccess) {
         bq_set_error("upload failed");
@@ -2010,7 +2187,16 @@ extern "C" void busseq_destroy(busseq_sampler* s)
     if (!s) return;
     (void)hipSetDevice(s->dev);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
     drop_graphs(s);
+    for (TraceHost& t : s->th)
+        for (int a = 0; a < BQ_TRACE_ARRAYS; ++a) {
+            if (t.stage[a]) (void)hipHostFree(t.stage[a]);
+            std::free(t.host[a]);
+        }
+    if (s->ev_sweeps) (void)hipEventDestroy(s->ev_sweeps);
+    for (hipEvent_t e : s->ev_copy) if (e) (void)hipEventDestroy(e);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->d_cd) (void)hipFree(s->d_cd);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -2029,7 +2215,6 @@ extern "C" busseq_status busseq_sweeps(busseq_sampler* s, int n)
     rc = launch_sweeps(s, n);
     if (rc != BUSSEQ_OK) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
-    s->iter += n;
     return BUSSEQ_OK;
 }
 
@@ -2053,7 +2238,6 @@ extern "C" busseq_status busseq_time_sweeps(busseq_sampler* s, int n, double* ms
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = f;
-    s->iter += n;
     return BUSSEQ_OK;
 }
 
@@ -2147,6 +2331,7 @@ extern "C" busseq_status busseq_kernel_times(busseq_sampler* s, int n, double* m
     HIPCHK(hipStreamSynchronize(s->stream));
     for (int it = 0; it < n; ++it) {
         busseq_status rc = s->prec == 32 ? enqueue_sweep<float>(s, ev.data()) : enqueue_sweep<double>(s, ev.data());
+        if (rc == BUSSEQ_OK) rc = count_sweep(s);
         if (rc != BUSSEQ_OK) return rc;
         HIPCHK(hipEventSynchronize(ev[kNumKernels]));
         for (int j = 0; j < kNumKernels; ++j) {
@@ -2157,7 +2342,6 @@ extern "C" busseq_status busseq_kernel_times(busseq_sampler* s, int n, double* m
     }
     for (int j = 0; j < kNumKernels; ++j) ms[j] /= n;
     for (auto& e : ev) (void)hipEventDestroy(e);
-    s->iter += n;
     return BUSSEQ_OK;
 }
 
@@ -2454,37 +2638,26 @@ extern "C" busseq_status busseq_finish(busseq_sampler* s, int ch, busseq_result*
     rc = busseq_bic(s, ch, r->alpha, r->beta, r->nu, r->delta, r->phi, r->gamma, r->pi, &r->bic);
     if (rc != BUSSEQ_OK) return rc;
     if (c.traces && r->n_pres > 0) {
-        // device traces are iteration-major ([iter][param], one coalesced row per
-        // recorded sweep); the result holds the recorded ones parameter-major
-        // ([param][iter]) with the stride it reports: after fewer sweeps than
-        // n_iter only the first nrec rows were written
-        const size_t P = (size_t)r->n_pres;
-        auto cpd = [&](double*& dst, const double* src, size_t k) -> hipError_t {
-            dst = md(k * P);
-            std::vector<double> tmp(k * P);
-            hipError_t e = hipMemcpy(tmp.data(), src, k * P * 8, hipMemcpyDeviceToHost);
-            for (size_t r = 0; r < P; ++r)
-                for (size_t j = 0; j < k; ++j) dst[j * P + r] = tmp[r * k + j];
-            return e;
-        };
-        auto cpi = [&](int32_t*& dst, const int32_t* src, size_t k) -> hipError_t {
-            dst = mi(k * P);
-            std::vector<int32_t> tmp(k * P);
-            hipError_t e = hipMemcpy(tmp.data(), src, k * P * 4, hipMemcpyDeviceToHost);
-            for (size_t r = 0; r < P; ++r)
-                for (size_t j = 0; j < k; ++j) dst[j * P + r] = tmp[r * k + j];
-            return e;
-        };
-        HIPCHK(cpd(r->tr_gamma, c.tr_gamma, 2 * (size_t)B));
-        HIPCHK(cpd(r->tr_alpha, c.tr_alpha, G));
-        HIPCHK(cpi(r->tr_L, c.tr_L, (size_t)(K - 1) * G));
-        HIPCHK(cpd(r->tr_beta, c.tr_beta, (size_t)(K - 1) * G));
-        HIPCHK(cpd(r->tr_ptau, c.tr_ptau, 2));
-        HIPCHK(cpd(r->tr_nu, c.tr_nu, (size_t)(B - 1) * G));
-        HIPCHK(cpd(r->tr_delta, c.tr_delta, N));
-        HIPCHK(cpd(r->tr_phi, c.tr_phi, (size_t)B * G));
-        HIPCHK(cpi(r->tr_W, c.tr_W, N));
-        HIPCHK(cpd(r->tr_pi, c.tr_pi, (size_t)B * K));
+        // the streamed host traces ([param * n_pres + iter]) with the stride the
+        // result reports: after fewer sweeps than n_iter only nrec were recorded
+        rc = trace_flush(s);
+        if (rc != BUSSEQ_OK) return rc;
+        const TraceHost& t = s->th[ch];
+        const size_t P = (size_t)r->n_pres, Pd = (size_t)t.P;
+        void** dst[BQ_TRACE_ARRAYS] = {(void**)&r->tr_gamma, (void**)&r->tr_alpha, (void**)&r->tr_L, (void**)&r->tr_beta,
+                                       (void**)&r->tr_ptau, (void**)&r->tr_nu, (void**)&r->tr_delta, (void**)&r->tr_phi,
+                                       (void**)&r->tr_W, (void**)&r->tr_pi};
+        for (int a = 0; a < BQ_TRACE_ARRAYS; ++a) {
+            const size_t es = (size_t)t.esz[a];
+            char* d = (char*)std::calloc(t.np[a] * P + 1, es);
+            *dst[a] = d;
+            if (!d) { bq_set_error("host allocation failed"); return BUSSEQ_ERR_NOMEM; }
+            const char* src = (const char*)t.host[a];
+            if (!src) continue;
+            if (P == Pd) std::memcpy(d, src, t.np[a] * P * es);
+            else
+                for (size_t j = 0; j < t.np[a]; ++j) std::memcpy(d + j * P * es, src + j * Pd * es, P * es);
+        }
     }
     return BUSSEQ_OK;
 }

@@ -600,3 +600,29 @@ extern "C" void busseq_result_free(busseq_result* r)
     for (void* q : ps) std::free(q);
     std::memset(r, 0, sizeof(*r));
 }
+
+// ---------------------------------------------------------------------------
+// streamed traces: one block of rows (iteration-major, as the device writes
+// them) into the parameter-major host traces ([param * P + iter], cu:892-936);
+// 256 parameters per task so that each task writes its rows' columns together
+// ---------------------------------------------------------------------------
+void bq_trace_transpose(const void* stage, int esz, size_t np, int rows, const int* col, size_t P, void* dst)
+{
+    const long ntile = (long)((np + 255) / 256);
+    #pragma omp parallel for schedule(static) if (np * (size_t)rows > 65536)
+    for (long tj = 0; tj < ntile; ++tj) {
+        const size_t j0 = (size_t)tj * 256, j1 = j0 + 256 < np ? j0 + 256 : np;
+        for (int q = 0; q < rows; ++q) {
+            if (col[q] < 0) continue;
+            if (esz == 8) {
+                const double* sr = (const double*)stage + (size_t)q * np;
+                double* d = (double*)dst + col[q];
+                for (size_t j = j0; j < j1; ++j) d[j * P] = sr[j];
+            } else {
+                const int32_t* sr = (const int32_t*)stage + (size_t)q * np;
+                int32_t* d = (int32_t*)dst + col[q];
+                for (size_t j = j0; j < j1; ++j) d[j * P] = sr[j];
+            }
+        }
+    }
+}

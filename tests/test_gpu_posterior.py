@@ -75,6 +75,35 @@ def test_traces_of_a_partial_run_match_the_oracle():
     np.testing.assert_allclose(post["traces"]["alpha"].mean(1), post["alpha"], rtol=1e-12)
 
 
+def test_streamed_traces_over_many_blocks_match_the_oracle():
+    # traces stream to host memory in blocks of 32 sweeps through a device ring of
+    # two blocks: runs that wrap the ring several times, two problems in one batch
+    # with different burn-in and run lengths (their blocks are recorded from
+    # different sweeps), and sweeps launched one kernel at a time (the per-kernel
+    # timing path) in the middle; every chain's traces equal its oracle's
+    specs = [([24, 20], 90, 3, 31, 150, 10), ([18, 16, 14], 70, 4, 32, 120, 53)]
+    probs, oracles = [], []
+    for n_b, G, K, ds, n_iter, n_burnin in specs:
+        Y = bq.simulate(n_b, G, K, ds)
+        kw = dict(seed=ds, n_iter=n_iter, n_burnin=n_burnin, n_unchanged=5)
+        probs.append(dict(Y=Y, n_b=n_b, K=K, keep_traces=True, **kw))
+        o = po.Oracle(Y, n_b, K, keep_traces=True, **kw)
+        o.sweeps(150)
+        oracles.append(o)
+    with bq.Sampler.batch(probs) as s:
+        s.sweeps(60)
+        s.kernel_times(7)        # sweeps 60..66: across the boundary at 64
+        s.sweeps(83)
+        assert s.iteration == 150
+        for j, (o, p) in enumerate(zip(oracles, probs)):
+            post = s.finish(j)
+            P = p["n_iter"] - p["n_burnin"]
+            assert post["n_pres"] == P
+            ot = o.traces()
+            for key in po.TRACE_KEYS:
+                assert np.array_equal(post["traces"][key], ot[key]), (j, key)
+
+
 def mc_se(tr, ess=False):
     """Monte-Carlo standard error of each row's mean, tau from Geyer's initial positive
     sequence; with ess=True also the effective sample size n / tau"""
