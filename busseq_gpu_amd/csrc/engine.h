@@ -46,7 +46,7 @@
     Q double *alpha, *beta, *nu, *delta, *phi, *gamma, *pi, *ptau; \
     Q double *mu_alpha, *mu_nu, *mu_delta; \
     /* precision-T copies of the parameters */ \
-    Q void *ea, *eb, *bt, *en, *ed, *ph;  /* bt: [G][bt_stride(K)][2] (beta_k, expm1 beta_k), k = 1..K-1 */ \
+    Q void *ea, *eb, *bt, *en, *ed, *ph;  /* bt: [G][bt_stride(K)][2] (beta_k, expm1 beta_k clamped as bq_tw_em_f32 in single precision), k = 1..K-1 */ \
     /* proposals and their precision-T constants */ \
     Q double *alpha_p, *beta_p, *nu_p, *phi_p, *gamma_p; \
     Q double *delta_p, *delta_lu;  /* [N] delta proposal and log of its accept uniform (k_prep) */ \

@@ -102,11 +102,16 @@ __device__ __forceinline__ V ld_at(const BQ_GLOBAL void* base, unsigned o)
     return *(const BQ_GLOBAL V*)((const BQ_GLOBAL char*)base + o);
 }
 
-// (beta_kg, expm1 beta_kg) pairs of the subtype draw, gene-major: gene g holds
+// (beta_kg, expm1 beta_kg) pairs of the subtype draw (expm1 clamped in single
+// precision, tw_em), gene-major: gene g holds
 // the pairs of k = 1..K-1 at pair index g * bt_stride(K) + k - 1; the stride is
 // K - 1 rounded up to even, so a gene's pairs start 16-byte aligned in single
 // precision and are read with one lane offset and immediate offsets.
 __host__ __device__ __forceinline__ int bt_stride(int K) { return K & ~1; }
+
+// expm1(beta) as the subtype draw reads it: clamped in single precision (bq_tw_em_f32)
+__host__ __device__ __forceinline__ float tw_em(double b, float) { return bq_tw_em_f32(b); }
+__host__ __device__ __forceinline__ double tw_em(double b, double) { return bq_expm1(b); }
 
 // value of lane k (a constant or wave-uniform index) in every lane
 __device__ __forceinline__ double read_lane(double v, int k)
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
     const BQ_GLOBAL char* ea = (const BQ_GLOBAL char*)c.ea;
     const BQ_GLOBAL char* en = (const BQ_GLOBAL char*)c.en + (size_t)b * G * sizeof(T);
     const BQ_GLOBAL char* ph = (const BQ_GLOBAL char*)c.ph + (size_t)b * G * sizeof(T);
-    const BQ_GLOBAL char* be = (const BQ_GLOBAL char*)c.bt;   // (beta_kg, expm1(beta_kg)) pairs
+    const BQ_GLOBAL char* be = (const BQ_GLOBAL char*)c.bt;   // (beta_kg, tw_em(beta_kg)) pairs
     const T edi = ((const T*)c.ed)[i];
     const BQ_GLOBAL char* row = (const BQ_GLOBAL char*)(c.S + (size_t)i * c.Gp);
 
@@ -1144,7 +1149,7 @@ __device__ __forceinline__ void beta_decide(const ChainG& c, int k, int g, int t
         ((T*)c.eb)[q] = (T)bq_exp(bp);
         const size_t pq = (size_t)g * bt_stride(K) + (k - 1);
         ((T*)c.bt)[2 * pq] = (T)bp;
-        ((T*)c.bt)[2 * pq + 1] = (T)bq_expm1(bp);
+        ((T*)c.bt)[2 * pq + 1] = tw_em(bp, T());
     }
 }
 
@@ -1796,7 +1801,7 @@ void to_pairs(std::vector<char>& out, const std::vector<double>& beta, int K, in
             const double v = beta[(size_t)k * G + g];
             const size_t q = (size_t)g * st + (k - 1);
             p[2 * q] = (T)v;
-            p[2 * q + 1] = (T)bq_expm1(v);
+            p[2 * q + 1] = tw_em(v, T());
         }
 }
 

@@ -114,12 +114,15 @@ BQ_DEFINE_ELEMENT_TERMS(double, f64, bq_log1p, bq_lgamma_diff64, __builtin_fma)
 BQ_FN double bq_tw_f64(double x, double phi, double r0, double beta, double em)
 { return __builtin_fma(-(phi + x), bq_log1p_mid(r0 * em), x * beta); }
 
-/* The single-precision subtype logit term: expm1(beta) is held to
- * [-1 + 2^-24, 2^100] (beta beyond about +-69 only), so r0 expm1(beta) with
- * 0 <= r0 <= 1 stays where the branch-free log1pf holds. */
-BQ_FN float bq_tw_f32(float x, float phi, float r0, float beta, float em)
+/* The single-precision subtype logit term takes expm1(beta) held to
+ * [-1 + 2^-24, 2^100] (it moves only beta beyond about +-69), so that r0
+ * expm1(beta) with 0 <= r0 <= 1 stays where the branch-free log1pf holds; the
+ * value is clamped once, where the parameter copy is written (bq_tw_em_f32),
+ * not per cell x gene. */
+BQ_FN float bq_tw_clamp_f32(float em) { return __builtin_fminf(__builtin_fmaxf(em, -0.99999994f), 0x1p100f); }
+BQ_FN float bq_tw_em_f32(double beta) { return bq_tw_clamp_f32((float)bq_expm1(beta)); }
+BQ_FN float bq_tw_f32(float x, float phi, float r0, float beta, float emc)
 {
-    const float emc = __builtin_fminf(__builtin_fmaxf(em, -0.99999994f), 0x1p100f);
     return __builtin_fmaf(-(phi + x), bq_log1pf_mid(r0 * emc), x * beta);
 }
 

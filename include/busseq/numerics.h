@@ -137,6 +137,11 @@ BQ_FN double bq_uniform(bq_rng* r)
 /* ------------------------------------------------------------------------ */
 #define BQ_LN2_HI 0.6931467056274414        /* 0x3fe62e4200000000, 21-bit mantissa */
 #define BQ_LN2_LO 4.7493250390316726e-07
+/* ln 2 (hi, lo) times 2^-52: the exponent k enters as the double (k << 52), which
+ * is exact, and k * ln2 is the same real product, so the same rounding, without
+ * the shift back */
+#define BQ_LN2_HI_S52 (BQ_LN2_HI * 0x1p-52)
+#define BQ_LN2_LO_S52 (BQ_LN2_LO * 0x1p-52)
 #define BQ_INV_LN2 1.4426950408889634
 #define BQ_HALF_LOG_2PI 0.9189385332046728
 #define BQ_LOG1P_LO (-0.29289321881345248)   /* sqrt(1/2) - 1 */
@@ -218,9 +223,9 @@ BQ_FN double bq_log1p_mid(double y)
     uint64_t kb = (bq_dbits(t) - 0x3FE6A09E667F3BCDull) & 0xFFF0000000000000ull;   /* k << 52 */
     double s = bq_bitsd(0x3FF0000000000000ull - kb);                                  /* 2^-k */
     double f = __builtin_fma(y, s, s - 1.0);
-    double ed = (double)((int64_t)kb >> 52);
+    double eds = (double)(int64_t)kb;   /* k 2^52 */
     double r = __builtin_fma(f * f, bq_log1p_q64(f), f);
-    return __builtin_fma(ed, BQ_LN2_HI, __builtin_fma(ed, BQ_LN2_LO, r));
+    return __builtin_fma(eds, BQ_LN2_HI_S52, __builtin_fma(eds, BQ_LN2_LO_S52, r));
 }
 
 /* (exp(r) - 1 - r) / r^2 on |r| <= ln2/2, degree 11, rel err 6e-19 */
@@ -347,6 +352,11 @@ BQ_FN double bq_lgamma(double z)
 /* ------------------------------------------------------------------------ */
 #define BQ_LN2_HI_F 0.693115234375f
 #define BQ_LN2_LO_F 3.194618329871446e-05f
+/* ln 2 (hi, lo) times 2^-23: the exponent k enters as the float (k << 23), which
+ * is exact, and k * ln2 is the same real product, so the same rounding, without
+ * the shift back */
+#define BQ_LN2_HI_F23 (BQ_LN2_HI_F * 0x1p-23f)
+#define BQ_LN2_LO_F23 (BQ_LN2_LO_F * 0x1p-23f)
 #define BQ_LOG1P_LO_F (-0.29289321f)
 #define BQ_LOG1P_HI_F 0.41421356f
 
@@ -365,13 +375,13 @@ BQ_FN float bq_log1p_q32(float f)
     return p;
 }
 
-BQ_FN float bq_split32(float x, int* e)
+/* split a positive finite normal x into 2^k * m, m in [sqrt(1/2), sqrt(2)); kb = k << 23 */
+BQ_FN float bq_split32s(float x, int32_t* kb)
 {
     uint32_t i = bq_fbits(x);
-    int32_t t = (int32_t)(i - 0x3F3504F3u);
-    int32_t k = t >> 23;
-    *e = (int)k;
-    return bq_bitsf(i - ((uint32_t)k << 23));
+    uint32_t k23 = (i - 0x3F3504F3u) & 0xFF800000u;
+    *kb = (int32_t)k23;
+    return bq_bitsf(i - k23);
 }
 
 /* branch-free: every lane runs the same instructions; special inputs are
@@ -380,12 +390,12 @@ BQ_FN float bq_logf(float x)
 {
     int sub = x < 1.17549435e-38f;
     float xs = sub ? x * 33554432.0f : x; /* 2^25 */
-    int e;
-    float m = bq_split32(xs, &e);
+    int32_t kb;
+    float m = bq_split32s(xs, &kb);
     float f = m - 1.0f;
     float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
-    float ef = (float)(e + (sub ? -25 : 0));
-    float res = __builtin_fmaf(ef, BQ_LN2_HI_F, __builtin_fmaf(ef, BQ_LN2_LO_F, r));
+    float efs = (float)(kb + (sub ? -(25 << 23) : 0));   /* k 2^23 */
+    float res = __builtin_fmaf(efs, BQ_LN2_HI_F23, __builtin_fmaf(efs, BQ_LN2_LO_F23, r));
     if (!(x > 0.0f)) res = x == 0.0f ? -BQ_INFF : BQ_NANF;
     if (x == BQ_INFF) res = BQ_INFF;
     return res;
@@ -396,12 +406,12 @@ BQ_FN float bq_logf(float x)
 /* logf for a positive, finite, normal argument (no special-value handling) */
 BQ_FN float bq_logf_norm(float x)
 {
-    int e;
-    float m = bq_split32(x, &e);
+    int32_t kb;
+    float m = bq_split32s(x, &kb);
     float f = m - 1.0f;
     float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
-    float ef = (float)e;
-    return __builtin_fmaf(ef, BQ_LN2_HI_F, __builtin_fmaf(ef, BQ_LN2_LO_F, r));
+    float efs = (float)kb;   /* k 2^23 */
+    return __builtin_fmaf(efs, BQ_LN2_HI_F23, __builtin_fmaf(efs, BQ_LN2_LO_F23, r));
 }
 
 BQ_FN float bq_log1pf_rare(float y)
@@ -425,9 +435,9 @@ BQ_FN float bq_log1pf(float y)
     uint32_t kb = (bq_fbits(t) - 0x3F3504F3u) & 0xFF800000u;   /* k << 23 */
     float s = bq_bitsf(0x3F800000u - kb);                        /* 2^-k */
     float f = __builtin_fmaf(y, s, s - 1.0f);
-    float ef = (float)((int32_t)kb >> 23);
+    float efs = (float)(int32_t)kb;   /* k 2^23 */
     float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
-    float res = __builtin_fmaf(ef, BQ_LN2_HI_F, __builtin_fmaf(ef, BQ_LN2_LO_F, r));
+    float res = __builtin_fmaf(efs, BQ_LN2_HI_F23, __builtin_fmaf(efs, BQ_LN2_LO_F23, r));
     /* 0 < t < sqrt(2) 2^125 (y > -1, k <= 125, not NaN) as one unsigned compare of the bits */
     if (bq_fbits(t) - 1u >= 0x7E3504F3u - 1u) res = bq_log1pf_rare(y);
     return res;
@@ -441,9 +451,9 @@ BQ_FN float bq_log1pf_mid(float y)
     uint32_t kb = (bq_fbits(t) - 0x3F3504F3u) & 0xFF800000u;
     float s = bq_bitsf(0x3F800000u - kb);
     float f = __builtin_fmaf(y, s, s - 1.0f);
-    float ef = (float)((int32_t)kb >> 23);
+    float efs = (float)(int32_t)kb;   /* k 2^23 */
     float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
-    return __builtin_fmaf(ef, BQ_LN2_HI_F, __builtin_fmaf(ef, BQ_LN2_LO_F, r));
+    return __builtin_fmaf(efs, BQ_LN2_HI_F23, __builtin_fmaf(efs, BQ_LN2_LO_F23, r));
 }
 
 /* Stirling remainder S(w) = lgamma(w) - (w - 1/2) log w + w - log(2 pi)/2 for
@@ -658,7 +668,8 @@ BQ_FN int bq_nb_inv_ok(double mu, double phi) { return mu <= 30.0 && 3.0 * mu <=
 BQ_FN int bq_rnbinom_inv_t(double u, double mu, double phi, const double* inv)
 {
     double q = mu / (phi + mu);
-    double p = bq_exp(phi * bq_log1p(-q));
+    /* -q in (-0.85, 0): often below bq_log1p's fast range, so the branch-free form */
+    double p = bq_exp(phi * bq_log1p_mid(-q));
     double s = p;
     int k = 0;
     double c = q * (phi * inv[1]);

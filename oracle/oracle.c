@@ -229,7 +229,8 @@ static void refresh_beta(orc_t* o, int k, int g)
 {
     size_t j = (size_t)k * o->G + g;
     SETT(eb, j, bq_exp(o->beta[j]));
-    SETT(em, j, bq_expm1(o->beta[j]));
+    if (o->prec == 32) ((float*)o->em)[j] = bq_tw_em_f32(o->beta[j]);   /* clamped for the subtype term */
+    else ((double*)o->em)[j] = bq_expm1(o->beta[j]);
     SETT(bt, j, o->beta[j]);
 }
 static void refresh_nu(orc_t* o, int b, int g) { SETT(en, (size_t)b * o->G + g, bq_exp(o->nu[(size_t)b * o->G + g])); }
@@ -1147,7 +1148,7 @@ void orc_terms(int fn, int prec, const double* a, int na, double* out, int n)
                                   bq_lgamma_diff64(f, d));
         } else if (prec == 32) {
             if (fn == 0) res = bq_tmh_f32((float)v[0], (float)v[1], (float)v[2], (float)v[3], (float)v[4]);
-            else if (fn == 1) res = bq_tw_f32((float)v[0], (float)v[1], (float)v[2], (float)v[3], (float)v[4]);
+            else if (fn == 1) res = bq_tw_f32((float)v[0], (float)v[1], (float)v[2], (float)v[3], bq_tw_clamp_f32((float)v[4]));
             else res = bq_ratio_f32((float)v[0], (float)v[1]);
         } else {
             if (fn == 0) res = bq_tmh_f64(v[0], v[1], v[2], v[3], v[4]);
