@@ -195,7 +195,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
-    ap.add_argument("--kernel-sweeps", type=int, default=5)
+    ap.add_argument("--kernel-sweeps", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")

@@ -2330,24 +2330,38 @@ extern "C" busseq_status busseq_kernel_times(busseq_sampler* s, int n, double* m
 {
     if (!s || n < 1 || !ms) { bq_set_error("bad arguments to busseq_kernel_times"); return BUSSEQ_ERR_ARG; }
     HIPCHK(hipSetDevice(s->dev));
-    std::vector<hipEvent_t> ev(kNumKernels + 1);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    // the n sweeps run back to back, as in the graphs, with one wait at the end
+    std::vector<hipEvent_t> ev((size_t)n * (kNumKernels + 1), nullptr);
+    busseq_status rc = BUSSEQ_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { e = nullptr; rc = BUSSEQ_ERR_DEVICE; break; }
+    if (rc != BUSSEQ_OK) bq_set_error("hipEventCreate failed in busseq_kernel_times");
     for (int j = 0; j < kNumKernels; ++j) ms[j] = 0.0;
-    HIPCHK(hipStreamSynchronize(s->stream));
-    for (int it = 0; it < n; ++it) {
-        busseq_status rc = s->prec == 32 ? enqueue_sweep<float>(s, ev.data()) : enqueue_sweep<double>(s, ev.data());
+    for (int it = 0; it < n && rc == BUSSEQ_OK; ++it) {
+        hipEvent_t* e = ev.data() + (size_t)it * (kNumKernels + 1);
+        rc = s->prec == 32 ? enqueue_sweep<float>(s, e) : enqueue_sweep<double>(s, e);
         if (rc == BUSSEQ_OK) rc = count_sweep(s);
-        if (rc != BUSSEQ_OK) return rc;
-        HIPCHK(hipEventSynchronize(ev[kNumKernels]));
+    }
+    if (rc == BUSSEQ_OK && hipStreamSynchronize(s->stream) != hipSuccess) {
+        bq_set_error("hipStreamSynchronize failed in busseq_kernel_times");
+        rc = BUSSEQ_ERR_DEVICE;
+    }
+    for (int it = 0; it < n && rc == BUSSEQ_OK; ++it) {
+        hipEvent_t* e = ev.data() + (size_t)it * (kNumKernels + 1);
         for (int j = 0; j < kNumKernels; ++j) {
             float f = 0.f;
-            HIPCHK(hipEventElapsedTime(&f, ev[j], ev[j + 1]));
+            if (hipEventElapsedTime(&f, e[j], e[j + 1]) != hipSuccess) {
+                bq_set_error("hipEventElapsedTime failed in busseq_kernel_times");
+                rc = BUSSEQ_ERR_DEVICE;
+                break;
+            }
             ms[j] += f;
         }
     }
     for (int j = 0; j < kNumKernels; ++j) ms[j] /= n;
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    return BUSSEQ_OK;
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 // streaming copy of known size; calibrates the HBM byte counters of a profiler

@@ -174,8 +174,9 @@ busseq_status busseq_autotune_batch(const busseq_problem* problems, int n_proble
 
 /* Measurement helpers (no counterpart in the reference, which has no timers).
  * busseq_kernel_times runs n sweeps as individual launches with HIP events
- * between them and returns the average milliseconds of each of the
- * busseq_kernel_count() kernels (names from busseq_kernel_name). */
+ * between them, back to back on the sampler's stream with one wait at the end,
+ * and returns the average milliseconds of each of the busseq_kernel_count()
+ * kernels (names from busseq_kernel_name). */
 int busseq_kernel_count(void);
 const char* busseq_kernel_name(int j);
 busseq_status busseq_kernel_times(busseq_sampler* s, int n, double* ms);
