@@ -100,6 +100,7 @@ def _load():
     lib.busseq_kernel_name.restype = _C.c_char_p
     lib.busseq_kernel_times.argtypes = [vp, _C.c_int, _PD]
     lib.busseq_copy_probe.argtypes = [_C.c_size_t, _C.c_int, _C.POINTER(_C.c_double)]
+    lib.busseq_rcp_check.argtypes = [_C.c_int, _C.POINTER(_C.c_uint64)]
     lib.busseq_autotune_batch.argtypes = [_C.POINTER(_Problem), _C.c_int, _C.c_int, _C.POINTER(_C.c_int),
                                           _C.POINTER(_C.c_double)]
     lib.busseq_simulate.argtypes = [_C.c_int, _C.POINTER(_C.c_int), _C.c_int, _C.c_int, _C.c_uint32,
@@ -161,6 +162,16 @@ def copy_probe(nbytes, device=0):
     ms = _C.c_double()
     _check(lib.busseq_copy_probe(int(nbytes), int(device), _C.byref(ms)))
     return ms.value
+
+
+def rcp_check(device=0):
+    """Number of normal floats z in [2^-126, 2^126) whose reciprocal on the
+    device differs from the host's, the IEEE quotient 1/z of z held to
+    [2^-126, 2^125]; single precision needs 0."""
+    lib = _load()
+    n = _C.c_uint64()
+    _check(lib.busseq_rcp_check(int(device), _C.byref(n)))
+    return n.value
 
 
 def read_dims(path, B):

@@ -1984,6 +1984,81 @@ busseq_status init_raw_sums(busseq_sampler* s, const int32_t* S0, int B, int N, 
 
 } // namespace
 
+// ---------------------------------------------------------------------------
+// bq_rcpf's precondition: on this device it returns what the host returns, the
+// IEEE quotient 1.0f / z of its argument held to [2^-126, 2^125], for every
+// normal positive z below 2^126 (the reciprocal and its Newton step for each
+// held value, and the hold itself).  One thread per float: exponent field
+// 1 + blockIdx.y (1..252), mantissa blockIdx.x * 256 + threadIdx.x (< 2^23).
+// ---------------------------------------------------------------------------
+#define BQ_RCP_GRID_Y 252
+__global__ __launch_bounds__(256) void k_rcp_check(unsigned long long* bad)
+{
+    const uint32_t e = 1u + blockIdx.y;
+    const float z = bq_bitsf((e << 23) | (blockIdx.x * 256u + threadIdx.x));
+    const float held = __builtin_fminf(__builtin_fmaxf(z, BQ_RCP_LO), BQ_RCP_HI);
+    const bool differ = bq_fbits(bq_rcpf(z)) != bq_fbits(1.0f / held);
+    const uint64_t m = __ballot(differ);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(bad, (unsigned long long)__popcll(m));
+}
+
+// number of the 252 x 2^23 floats for which bq_rcpf(z) differs from the host's
+// value on the current device
+static busseq_status rcp_count(unsigned long long* bad)
+{
+    unsigned long long* d = nullptr;
+    if (hipMalloc((void**)&d, sizeof *d) != hipSuccess) {
+        bq_set_error("hipMalloc failed in the reciprocal check");
+        return BUSSEQ_ERR_NOMEM;
+    }
+    hipError_t e = hipMemset(d, 0, sizeof *d);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_rcp_check, dim3(1u << 15, BQ_RCP_GRID_Y), dim3(256), 0, 0, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(bad, d, sizeof *bad, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        bq_set_error(std::string("HIP error in the reciprocal check: ") + hipGetErrorString(e));
+        return BUSSEQ_ERR_DEVICE;
+    }
+    return BUSSEQ_OK;
+}
+
+// once per device and process, before its first sampler: the single-precision
+// terms rely on bq_rcpf returning the host's quotient
+#define BQ_RCP_DEVICES 64
+static std::mutex g_rcp_mu;
+static int g_rcp_state[BQ_RCP_DEVICES];   // 0 not checked, 1 equal everywhere, -1 differs
+
+static busseq_status rcp_ready(int dev)
+{
+    if (dev < 0 || dev >= BQ_RCP_DEVICES) { bq_set_error("device index out of range"); return BUSSEQ_ERR_ARG; }
+    std::lock_guard<std::mutex> lock(g_rcp_mu);
+    if (g_rcp_state[dev] == 0) {
+        unsigned long long bad = 0;
+        busseq_status rc = rcp_count(&bad);
+        if (rc != BUSSEQ_OK) return rc;
+        g_rcp_state[dev] = bad == 0 ? 1 : -1;
+    }
+    if (g_rcp_state[dev] < 0) {
+        bq_set_error("this device's v_rcp_f32 with one Newton step differs from the IEEE quotient 1/z for "
+                     "some normal z in [2^-126, 2^125]; single-precision sweeps would not match the host's numerics");
+        return BUSSEQ_ERR_DEVICE;
+    }
+    return BUSSEQ_OK;
+}
+
+extern "C" busseq_status busseq_rcp_check(int device, uint64_t* mismatches)
+{
+    if (!mismatches) { bq_set_error("null output of busseq_rcp_check"); return BUSSEQ_ERR_ARG; }
+    if (hipSetDevice(device) != hipSuccess) { bq_set_error("hipSetDevice failed (no GPU?)"); return BUSSEQ_ERR_DEVICE; }
+    unsigned long long bad = 0;
+    busseq_status rc = rcp_count(&bad);
+    if (rc == BUSSEQ_OK) *mismatches = bad;
+    return rc;
+}
+
 extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_probs, busseq_sampler** out)
 {
     if (!out || !probs || n_probs < 1) { bq_set_error("bad arguments to busseq_create_batch"); return BUSSEQ_ERR_ARG; }
@@ -2005,6 +2080,10 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
     s->TR = c0->tile_rows > 0 ? c0->tile_rows : 64;
     s->dev = c0->device;
     if (hipSetDevice(s->dev) != hipSuccess) { bq_set_error("hipSetDevice failed (no GPU?)"); return fail(BUSSEQ_ERR_DEVICE); }
+    if (s->prec == 32) {
+        busseq_status rc = rcp_ready(s->dev);
+        if (rc != BUSSEQ_OK) return fail(rc);
+    }
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         bq_set_error("hipStreamCreate failed");
         return fail(BUSSEQ_ERR_DEVICE);

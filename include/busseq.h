@@ -183,6 +183,13 @@ busseq_status busseq_kernel_times(busseq_sampler* s, int n, double* ms);
 /* streaming copies of `bytes` bytes on the device: one warm-up launch, then *ms = the
  * average of ten launches (profiler counter calibration, achievable bandwidth) */
 busseq_status busseq_copy_probe(size_t bytes, int device, double* ms);
+/* Numerics self-check (no counterpart in the reference): *mismatches = the
+ * number of normal floats z in [2^-126, 2^126) (252 x 2^23 values)
+ * for which the device's reciprocal-plus-Newton form differs from the IEEE
+ * quotient 1.0f / z of z held to [2^-126, 2^125], the host's value.  Single-precision samplers depend on that number being
+ * 0; busseq_create/busseq_create_batch run the same check once per device and
+ * process and fail with BUSSEQ_ERR_DEVICE otherwise. */
+busseq_status busseq_rcp_check(int device, uint64_t* mismatches);
 
 /* message of the last failing call on this thread */
 const char* busseq_last_error(void);

@@ -360,6 +360,31 @@ BQ_FN double bq_lgamma(double z)
 #define BQ_LOG1P_LO_F (-0.29289321f)
 #define BQ_LOG1P_HI_F 0.41421356f
 
+/* 1/z for a positive z held to [2^-126, 2^125] (one v_med3_f32), rounded to
+ * nearest as the IEEE division 1.0f / z returns it.  On gfx950 the hardware
+ * reciprocal y = v_rcp_f32(z) followed by one Newton step
+ * fma(fma(-z, y, 1), y, y) (Markstein's final correction) gives that value for
+ * every normal z in [2^-126, 2^126): all 2113929216 positive ones were compared
+ * with the division on the device, and the library repeats that comparison on
+ * each device before its first single-precision sampler is created
+ * (busseq_rcp_check), so that host and device keep returning the same bits.
+ * The host takes the division of the same held value.  Four instructions
+ * instead of the division's eleven.  z is a positive finite value in every
+ * caller (phi + mu, (phi' + mu)(phi + x)(phi' + x)). */
+#define BQ_RCP_LO 0x1p-126f
+#define BQ_RCP_HI 0x1p125f
+BQ_FN float bq_rcpf(float z)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    const float zc = __builtin_amdgcn_fmed3f(z, BQ_RCP_LO, BQ_RCP_HI);   /* = the host's min(max()) for non-NaN z */
+    const float y = __builtin_amdgcn_rcpf(zc);
+    return __builtin_fmaf(__builtin_fmaf(-zc, y, 1.0f), y, y);
+#else
+    const float zc = __builtin_fminf(__builtin_fmaxf(z, BQ_RCP_LO), BQ_RCP_HI);
+    return 1.0f / zc;
+#endif
+}
+
 /* (log1p(f) - f) / f^2 on [sqrt(1/2)-1, sqrt(2)-1], degree 8, rel err 1e-8 */
 BQ_FN float bq_log1p_q32(float f)
 {

@@ -297,3 +297,30 @@ def test_planted_mistake_is_detected():
     # a shifted normal (a plausible sign/offset slip) must fail the same KS test
     x = draws("normal", 100000) + 0.02
     assert st.kstest(x, "norm").pvalue < 1e-3
+
+
+def test_single_precision_ratio_is_within_two_roundings_and_never_above_one():
+    # r = mu (1 / (phi + mu)) in single precision: within 2^-22 of the exact
+    # mu / (phi + mu) of the float inputs, and held to at most 1, which the
+    # subtype term needs (r expm1(beta) > -1 for expm1(beta) >= -1 + 2^-24)
+    rng = np.random.default_rng(17)
+    n = 200000
+    mu = np.exp(rng.uniform(np.log(1e-30), np.log(1e30), n)).astype(np.float32)
+    phi = np.exp(rng.uniform(np.log(1e-6), np.log(1e6), n)).astype(np.float32)
+    phi[:2000] = mu[:2000] * np.float32(1e-9)   # phi negligible: phi + mu == mu, r rounds to 1 or just below
+    phi[2000:2100] = np.float32(1e-38)
+    mu[2100:2200] = np.float32(3e38)            # phi + mu beyond the reciprocal's hold at 2^125
+    r = po.terms("ratio", 32, np.stack([mu.astype(np.float64), phi.astype(np.float64)], 1))
+    exact = mu.astype(np.float64) / (phi.astype(np.float64) + mu.astype(np.float64))
+    assert r.max() <= 1.0
+    ok = mu < 2.0 ** 124
+    assert np.all(np.abs(r[ok] - exact[ok]) <= 2.0 ** -22 * exact[ok] + 1e-45)
+    assert np.all(r[~ok] == 1.0)
+    # the subtype term stays finite at the smallest held expm1(beta) with r = 1
+    em = np.full(2000, -1 + 2.0 ** -24)
+    tw = po.terms("tw", 32, np.stack([np.full(2000, 3.0), phi[:2000].astype(np.float64), r[:2000],
+                                      np.log1p(em), em], 1))
+    assert np.isfinite(tw).all()
+    # planted mistake: without the minimum, mu beyond the reciprocal's hold gives r > 1
+    unheld = mu[2100:2200] * (np.float32(1) / np.float32(2.0 ** 125))
+    assert unheld.min() > 1.0
