@@ -277,8 +277,9 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
     if (i >= c.N) return;
     const int t = *c.iter;
     const int G = c.G, b = c.batch[i];
-    const BQ_GLOBAL char* ea = (const BQ_GLOBAL char*)c.ea;
-    const BQ_GLOBAL char* en = (const BQ_GLOBAL char*)c.en + (size_t)b * G * sizeof(T);
+    // subtype 0's mean factors (e^alpha e^beta_0) e^nu_b with e^beta_0 = 1: the
+    // first two products of tmu(e^alpha, 1, e^nu, e^delta), from the X/Z table
+    const BQ_GLOBAL char* t0 = (const BQ_GLOBAL char*)c.tm + (size_t)b * G * sizeof(T);
     const BQ_GLOBAL char* ph = (const BQ_GLOBAL char*)c.ph + (size_t)b * G * sizeof(T);
     const BQ_GLOBAL char* be = (const BQ_GLOBAL char*)c.bt;   // (beta_kg, tw_em(beta_kg)) pairs
     const T edi = ((const T*)c.ed)[i];
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
             const BQ_GLOBAL T* bg = (const BQ_GLOBAL T*)(be + g * (unsigned)(bt_stride(KT) * sizeof(T2)));
 #pragma unroll
             for (int k = 1; k < KT; ++k) { v[k].x = bg[2 * k - 2]; v[k].y = bg[2 * k - 1]; }
-            T mu0 = tmu(*(const BQ_GLOBAL T*)(ea + o), (T)1, *(const BQ_GLOBAL T*)(en + o), edi);
+            T mu0 = *(const BQ_GLOBAL T*)(t0 + o) * edi;   // tmu's last product
             T r0 = tratio(mu0, phi);
 #pragma unroll
             for (int k = 1; k < KT; ++k) part[k] += (double)ttw(x, phi, r0, v[k].x, v[k].y);

@@ -101,7 +101,7 @@ def main():
         return (1 + x + x * x * e - mp.exp(x)) / mp.exp(x)
 
     out = []
-    for name, cast, deg in (("LOG1P_Q_F32", f32, 8), ("LOG1P_Q_F64", f64, 19)):
+    for name, cast, deg in (("LOG1P_Q_F32", f32, 8), ("LOG1P_Q_F32_MID", f32, 7), ("LOG1P_Q_F64", f64, 19)):
         co, w = report(name, log1p_q, lo, hi, deg, cast, rel_log)
         out.append((name, co, w))
     r = mp.log(2) / 2 + mp.mpf("1e-3")

@@ -400,6 +400,22 @@ BQ_FN float bq_log1p_q32(float f)
     return p;
 }
 
+/* the same on the same interval, degree 7, rel err 5.6e-8: for bq_log1pf_mid,
+ * one fused multiply-add fewer in each of the subtype draw's K - 1 logarithms
+ * per cell x gene */
+BQ_FN float bq_log1p_q32_mid(float f)
+{
+    float p = 8.79710465669631958008e-2f;
+    p = __builtin_fmaf(p, f, -1.41752481460571289063e-1f);
+    p = __builtin_fmaf(p, f, 1.48381710052490234375e-1f);
+    p = __builtin_fmaf(p, f, -1.65869489312171936035e-1f);
+    p = __builtin_fmaf(p, f, 1.99716746807098388672e-1f);
+    p = __builtin_fmaf(p, f, -2.50011831521987915039e-1f);
+    p = __builtin_fmaf(p, f, 3.33336770534515380859e-1f);
+    p = __builtin_fmaf(p, f, -4.99999970197677612305e-1f);
+    return p;
+}
+
 /* split a positive finite normal x into 2^k * m, m in [sqrt(1/2), sqrt(2)); kb = k << 23 */
 BQ_FN float bq_split32s(float x, int32_t* kb)
 {
@@ -469,7 +485,8 @@ BQ_FN float bq_log1pf(float y)
 }
 
 /* log1p(y) for y in (-1, sqrt(2) 2^125), and NaN for NaN: bq_log1pf without
- * its branch to the special values, for callers that keep y in that range */
+ * its branch to the special values, for callers that keep y in that range,
+ * and with the degree-7 polynomial (within 1.5 ulp instead of 0.8) */
 BQ_FN float bq_log1pf_mid(float y)
 {
     float t = 1.0f + y;
@@ -477,7 +494,7 @@ BQ_FN float bq_log1pf_mid(float y)
     float s = bq_bitsf(0x3F800000u - kb);
     float f = __builtin_fmaf(y, s, s - 1.0f);
     float efs = (float)(int32_t)kb;   /* k 2^23 */
-    float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
+    float r = __builtin_fmaf(f * f, bq_log1p_q32_mid(f), f);
     return __builtin_fmaf(efs, BQ_LN2_HI_F23, __builtin_fmaf(efs, BQ_LN2_LO_F23, r));
 }
 

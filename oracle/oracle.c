@@ -1085,7 +1085,8 @@ void orc_nb_inv(const double* u, const double* mu, const double* phi, double* ou
 }
 
 /* fn: 0 log, 1 log1p, 2 exp, 3 expm1, 4 lgamma, 5 lgamma_diff64(x,y), 6 logf, 7 log1pf,
- *     8 lgamma_diff32(x,y), 9 softplus, 10 ndtri, 11 fix/unfix round trip, 12 log1p_mid */
+ *     8 lgamma_diff32(x,y), 9 softplus, 10 ndtri, 11 fix/unfix round trip, 12 log1p_mid,
+ *     13 log1pf_mid */
 void orc_math(int fn, const double* x, const double* y, double* out, int n)
 {
     for (int j = 0; j < n; ++j) {
@@ -1103,6 +1104,7 @@ void orc_math(int fn, const double* x, const double* y, double* out, int n)
         case 9: out[j] = bq_softplus(v); break;
         case 10: out[j] = bq_ndtri(v); break;
         case 12: out[j] = bq_log1p_mid(v); break;
+        case 13: out[j] = (double)bq_log1pf_mid((float)v); break;
         default: out[j] = bq_unfix(bq_fix(v)); break;
         }
     }

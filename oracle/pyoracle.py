@@ -69,7 +69,8 @@ POST_KEYS = ("gamma", "alpha", "L", "beta", "ptau", "nu", "delta", "phi", "pi", 
 
 # names of orc_math functions
 MATH = {"log": 0, "log1p": 1, "exp": 2, "expm1": 3, "lgamma": 4, "lgamma_diff64": 5, "logf": 6,
-        "log1pf": 7, "lgamma_diff32": 8, "softplus": 9, "ndtri": 10, "fix": 11, "log1p_mid": 12}
+        "log1pf": 7, "lgamma_diff32": 8, "softplus": 9, "ndtri": 10, "fix": 11, "log1p_mid": 12,
+        "log1pf_mid": 13}
 STREAM = {"uniform": 0, "normal": 1, "gamma": 2, "poisson": 3, "nbinom": 4, "beta": 5, "u32": 6,
           "nbinom_xz": 7}
 

@@ -181,6 +181,25 @@ def test_single_precision_log1p_over_its_reduction_ranges():
     assert ulps(po.math("log1pf", y.astype(np.float64)), ref, np.float32).max() <= 2.0
 
 
+def test_single_precision_log1p_mid_over_its_reduction_ranges():
+    # the branch-free log1pf of the subtype term, degree-7 polynomial: within 1.5 ulp
+    # for every exponent k of 1 + y from -24 to 125, and y tiny (f = y exactly)
+    import mpmath as mp
+    rng = np.random.default_rng(11)
+    y = np.concatenate([
+        -1 + np.exp2(-rng.uniform(0, 24, 400)),
+        np.exp2(rng.uniform(-40, 125.4, 1200)),
+        -np.exp2(rng.uniform(-40, -0.1, 600)),
+        rng.uniform(np.sqrt(0.5) - 1, np.sqrt(2) - 1, 3000),
+    ]).astype(np.float32)
+    y = y[np.isfinite(y) & (y > -1)]
+    ref = np.array([float(mp.log1p(mp.mpf(float(v)))) for v in y])
+    err = ulps(po.math("log1pf_mid", y.astype(np.float64)), ref, np.float32)
+    assert err.max() <= 1.5
+    # the degree-8 polynomial of log1pf stays within its tighter bound on the same inputs
+    assert ulps(po.math("log1pf", y.astype(np.float64)), ref, np.float32).max() <= 1.0
+
+
 def test_ndtri_vs_mpmath(gold):
     y = gold["ndtri_y"]
     err = np.abs(po.math("ndtri", gold["ndtri_x"]) - y) / np.maximum(np.abs(y), 1.0)
