@@ -59,7 +59,7 @@ __device__ __forceinline__ float tmu(float a, float b, float c, float d) { retur
 __device__ __forceinline__ double tmu(double a, double b, double c, double d) { return bq_mu_f64(a, b, c, d); }
 __device__ __forceinline__ float tratio(float m, float p) { return bq_ratio_f32(m, p); }
 __device__ __forceinline__ double tratio(double m, double p) { return bq_ratio_f64(m, p); }
-__device__ __forceinline__ float tlog1p(float x) { return bq_log1pf(x); }
+__device__ __forceinline__ float tlog1p(float x) { return bq_log1p_mh_f32(x); }
 __device__ __forceinline__ double tlog1p(double x) { return bq_log1p(x); }
 __device__ __forceinline__ float ttw(float x, float p, float r, float b, float e) { return bq_tw_f32(x, p, r, b, e); }
 __device__ __forceinline__ double ttw(double x, double p, double r, double b, double e) { return bq_tw_f64(x, p, r, b, e); }

@@ -111,7 +111,15 @@ BQ_FN T bq_tphi_gen_##SFX(int xi, T x, T phi, T phip, T mu, T d, T dop, T iphip,
  * at 2^125. */
 BQ_FN float bq_ratio_f32(float mu, float phi) { return __builtin_fminf(mu * bq_rcpf(phi + mu), 1.0f); }
 BQ_FN double bq_ratio_f64(double mu, double phi) { return mu / (phi + mu); }
-BQ_DEFINE_ELEMENT_TERMS(float, f32, bq_log1pf, bq_lgamma_diff32, __builtin_fmaf)
+/* The single-precision random-walk terms' logarithm log1p(r c): r = mu/(phi + mu)
+ * in [0, 1] and c = expm1 of one proposal step (alpha, beta, nu, delta) or of
+ * two (beta and nu together).  A step is BQ_RW_SD times a normal deviate of
+ * bq_normal, whose magnitude is at most ndtri(2^-33) = 6.34 (it inverts a
+ * uniform in [2^-33, 1 - 2^-33]), so |step| <= 0.634 and r c lies in
+ * (-0.72, 2.56): inside bq_log1pf_mid's range, which needs no branch. */
+BQ_FN float bq_log1p_mh_f32(float y) { return bq_log1pf_mid(y); }
+
+BQ_DEFINE_ELEMENT_TERMS(float, f32, bq_log1p_mh_f32, bq_lgamma_diff32, __builtin_fmaf)
 BQ_DEFINE_ELEMENT_TERMS(double, f64, bq_log1p, bq_lgamma_diff64, __builtin_fma)
 
 /* The double-precision subtype logit and phi terms take the branch-free

@@ -486,7 +486,8 @@ BQ_FN float bq_log1pf(float y)
 
 /* log1p(y) for y in (-1, sqrt(2) 2^125), and NaN for NaN: bq_log1pf without
  * its branch to the special values, for callers that keep y in that range,
- * and with the degree-7 polynomial (within 1.5 ulp instead of 0.8) */
+ * and with the degree-7 polynomial: within 2 ulp as bq_log1pf is, mean error
+ * 0.3 ulp instead of 0.25 */
 BQ_FN float bq_log1pf_mid(float y)
 {
     float t = 1.0f + y;

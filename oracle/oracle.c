@@ -668,7 +668,7 @@ static void pass_phi_##SFX(orc_t* o)                                            
     free(cst);                                                                             \
 }
 
-#define LOG1P_f32 bq_log1pf
+#define LOG1P_f32 bq_log1p_mh_f32
 #define LOG1P_f64 bq_log1p
 DEFINE_PASSES(float, f32)
 DEFINE_PASSES(double, f64)
