@@ -445,15 +445,18 @@ BQ_FN float bq_logf(float x)
 
 /* log1p for the arguments the fast path does not take: y <= -1, NaN, +inf and
  * 1 + y >= sqrt(2) 2^125 (k >= 126), where 1 + y equals y up to a relative 2^-125 */
-/* logf for a positive, finite, normal argument (no special-value handling) */
+/* logf for a positive, finite, normal argument (no special-value handling), for
+ * the phi term's lgamma difference: the degree-7 polynomial and k ln 2 in one
+ * fused multiply-add, as bq_log1pf (bq_logf keeps the degree-8 polynomial and
+ * the two-part ln 2, within 1 ulp) */
 BQ_FN float bq_logf_norm(float x)
 {
     int32_t kb;
     float m = bq_split32s(x, &kb);
     float f = m - 1.0f;
-    float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
+    float r = __builtin_fmaf(f * f, bq_log1p_q32_mid(f), f);
     float efs = (float)kb;   /* k 2^23 */
-    return __builtin_fmaf(efs, BQ_LN2_HI_F23, __builtin_fmaf(efs, BQ_LN2_LO_F23, r));
+    return __builtin_fmaf(efs, BQ_LN2_F23, r);
 }
 
 BQ_FN float bq_log1pf_rare(float y)
@@ -469,7 +472,9 @@ BQ_FN float bq_log1pf_rare(float y)
  * is formed from the unrounded y with a single rounding (2^-k - 1 is exact
  * for the k that occur here or carries an error far below f's own), so no
  * correction term for the rounding of 1 + y is needed, and for |y| small
- * (k = 0) f = y exactly.
+ * (k = 0) f = y exactly.  The degree-7 polynomial of bq_log1p_q32_mid and
+ * k ln 2 in one fused multiply-add with ln 2 rounded to single precision
+ * (1.9e-9 |k|, at most 0.13 ulp, exact for k = 0): within 2 ulp.
  */
 BQ_FN float bq_log1pf(float y)
 {
@@ -478,17 +483,17 @@ BQ_FN float bq_log1pf(float y)
     float s = bq_bitsf(0x3F800000u - kb);                        /* 2^-k */
     float f = __builtin_fmaf(y, s, s - 1.0f);
     float efs = (float)(int32_t)kb;   /* k 2^23 */
-    float r = __builtin_fmaf(f * f, bq_log1p_q32(f), f);
-    float res = __builtin_fmaf(efs, BQ_LN2_HI_F23, __builtin_fmaf(efs, BQ_LN2_LO_F23, r));
+    float r = __builtin_fmaf(f * f, bq_log1p_q32_mid(f), f);
+    float res = __builtin_fmaf(efs, BQ_LN2_F23, r);
     /* 0 < t < sqrt(2) 2^125 (y > -1, k <= 125, not NaN) as one unsigned compare of the bits */
     if (bq_fbits(t) - 1u >= 0x7E3504F3u - 1u) res = bq_log1pf_rare(y);
     return res;
 }
 
 /* log1p(y) for y in (-1, sqrt(2) 2^125), and NaN for NaN: bq_log1pf without
- * its branch to the special values, for callers that keep y in that range,
- * and with the degree-7 polynomial: within 2 ulp as bq_log1pf is, mean error
- * 0.3 ulp instead of 0.25 */
+ * its branch to the special values, for callers that keep y in that range
+ * (within 2 ulp; the degree-7 polynomial's mean error is 0.3 ulp, the
+ * degree-8 one's 0.25) */
 BQ_FN float bq_log1pf_mid(float y)
 {
     float t = 1.0f + y;
