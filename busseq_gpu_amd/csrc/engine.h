@@ -9,8 +9,11 @@
 #include <stdint.h>
 #include <vector>
 
-#define BQ_KMAX 16     // largest K supported (register arrays in the row kernels)
+#define BQ_KMAX 64     // largest K (the subtype draw keeps subtype k's logit in lane k)
+#define BQ_KREG 16     // subtypes per register group: the K <= 16 subtype-draw instantiations,
+                       // the K > 16 draw's groups, and the chunks of the cells' partition
 #define BQ_BMAX 128    // largest B (the reference's __constant__ limit, cu:9-10)
+#define BQ_BKMAX 2048  // largest B * K (the partition's tables in shared memory)
 #define BQ_WAVE 64
 #define BQ_CHUNK 1024 // genes per row-sum chunk (part of the reduction contract)
 #define BQ_TR_MAX 256  // largest tile (cells per tile-pass workgroup)

@@ -346,6 +346,85 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_w_wide: the subtype draw for BQ_KREG < K <= BQ_KMAX.  The same logits
+// and the same draw as k_w (each subtype's sum keeps its per-lane, per-chunk
+// grouping), computed for BQ_KREG subtypes at a time, one pass over the cell's
+// row per group; the logit of subtype k ends in lane k, as in k_w.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_w_wide(const ChainDev* __restrict__ cds)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    const int lane = threadIdx.x & 63;
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= c.N) return;
+    const int t = *c.iter;
+    const int G = c.G, K = c.K, b = c.batch[i];
+    const BQ_GLOBAL char* t0 = (const BQ_GLOBAL char*)c.tm + (size_t)b * G * sizeof(T);
+    const BQ_GLOBAL char* ph = (const BQ_GLOBAL char*)c.ph + (size_t)b * G * sizeof(T);
+    const BQ_GLOBAL T* be = (const BQ_GLOBAL T*)c.bt;
+    const unsigned pst = 2u * (unsigned)bt_stride(K);   // T values per gene in the pair table
+    const T edi = ((const T*)c.ed)[i];
+    const BQ_GLOBAL char* row = (const BQ_GLOBAL char*)(c.S + (size_t)i * c.Gp);
+    uint64_t mine = 0;
+    for (int k0 = 1; k0 < K; k0 += BQ_KREG) {
+        const int nk = K - k0 < BQ_KREG ? K - k0 : BQ_KREG;   // subtypes k0 .. k0 + nk - 1
+        uint64_t acc[BQ_KREG];
+#pragma unroll
+        for (int j = 0; j < BQ_KREG; ++j) acc[j] = 0;
+        for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
+            const unsigned c1 = (unsigned)(c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G);
+            double part[BQ_KREG];
+#pragma unroll
+            for (int j = 0; j < BQ_KREG; ++j) part[j] = 0.0;
+            for (unsigned g = (unsigned)(c0 + lane); g < c1; g += 64) {
+                const unsigned o = g * (unsigned)sizeof(T);
+                int32_t s = *(const BQ_GLOBAL int32_t*)(row + g * 4u);
+                int xi = s < 0 ? ~s : s;
+                T x = (T)xi;
+                T phi = *(const BQ_GLOBAL T*)(ph + o);
+                const BQ_GLOBAL T* bg = be + g * pst + 2u * (unsigned)(k0 - 1);
+                T mu0 = *(const BQ_GLOBAL T*)(t0 + o) * edi;   // tmu's last product
+                T r0 = tratio(mu0, phi);
+#pragma unroll
+                for (int j = 0; j < BQ_KREG; ++j)
+                    if (j < nk) part[j] += (double)ttw(x, phi, r0, bg[2 * j], bg[2 * j + 1]);
+            }
+#pragma unroll
+            for (int j = 0; j < BQ_KREG; ++j)
+                if (j < nk) acc[j] += bq_fix(part[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < BQ_KREG; ++j) {
+            if (j < nk) {
+                const uint64_t a = wave_sum_u64(acc[j]);
+                if (lane == k0 + j) mine = a;
+            }
+        }
+    }
+    double lpl = -BQ_INF;
+    if (lane < K) lpl = bq_unfix(mine) + bq_log(c.pi[b * K + lane]);
+    double mx = -BQ_INF;
+    for (int k = 0; k < K; ++k) {
+        const double v = read_lane(lpl, k);
+        if (v > mx) mx = v;
+    }
+    const double el = bq_exp(lpl - mx);
+    double tot = 0.0;
+    for (int k = 0; k < K; ++k) tot += read_lane(el, k);
+    double tgt = ph_uniform(c, (uint32_t)i, 0u, t, BQ_PH_W) * tot, cum = 0.0;
+    int kk = K - 1;
+    for (int k = 0; k < K; ++k) {
+        cum += read_lane(el, k);
+        if (tgt <= cum) { kk = k; break; }
+    }
+    if (lane == 0) {
+        c.W[i] = kk;
+        gatomic_add(&c.cnt[(size_t)(blockIdx.x % BQ_STRIPES) * c.B * K + b * K + kk], 1);
+    }
+}
+
 template <typename T>
 void launch_w(int K, dim3 grid, dim3 blk, hipStream_t st, const ChainDev* cd)
 {
@@ -355,7 +434,7 @@ void launch_w(int K, dim3 grid, dim3 blk, hipStream_t st, const ChainDev* cd)
         BQ_W_CASE(9) BQ_W_CASE(10) BQ_W_CASE(11) BQ_W_CASE(12) BQ_W_CASE(13) BQ_W_CASE(14)
         BQ_W_CASE(15) BQ_W_CASE(16)
 #undef BQ_W_CASE
-    default: break;
+    default: hipLaunchKernelGGL(k_w_wide<T>, grid, blk, 0, st, cd); break;
     }
 }
 
@@ -367,14 +446,14 @@ void launch_w(int K, dim3 grid, dim3 blk, hipStream_t st, const ChainDev* cd)
 // ints of the caller's shared memory.
 // ---------------------------------------------------------------------------
 #define PLAN_THREADS 256
-#define PLAN_LDS_INTS (2 * BQ_BMAX * BQ_KMAX + 1 + (PLAN_THREADS / 64) * BQ_KMAX)
+#define PLAN_LDS_INTS (2 * BQ_BKMAX + 1 + (PLAN_THREADS / 64) * BQ_KREG)
 __device__ __forceinline__ void plan_block(const ChainG& c, int b, int* lds)
 {
     if (b >= c.B) return;
     const int K = c.K, B = c.B, TR = c.TR, tid = threadIdx.x;
-    int* s_toff = lds;                                   // BQ_BMAX * BQ_KMAX + 1
-    int* s_cnt = lds + BQ_BMAX * BQ_KMAX + 1;            // BQ_BMAX * BQ_KMAX
-    int (*s_wsum)[BQ_KMAX] = (int (*)[BQ_KMAX])(s_cnt + BQ_BMAX * BQ_KMAX);
+    int* s_toff = lds;                                   // BQ_BKMAX + 1
+    int* s_cnt = lds + BQ_BKMAX + 1;                     // BQ_BKMAX
+    int (*s_wsum)[BQ_KREG] = (int (*)[BQ_KREG])(s_cnt + BQ_BKMAX);
     for (int j = tid; j < B * K; j += PLAN_THREADS) {
         s_cnt[j] = sum_stripes<int>((const int32_t*)c.cnt + j, (size_t)B * K);
     }
@@ -403,63 +482,67 @@ __device__ __forceinline__ void plan_block(const ChainG& c, int b, int* lds)
         }
         gstart += n;
     }
-    // stable partition: thread tid owns cells [lo, hi) of the batch
+    // stable partition: thread tid owns cells [lo, hi) of the batch; the
+    // subtypes are taken BQ_KREG at a time (one pass over the owned cells each)
     const int chunk = (nb + PLAN_THREADS - 1) / PLAN_THREADS;
     const int lo = b0 + tid * chunk;
     const int hi = lo + chunk < b0 + nb ? lo + chunk : b0 + nb;
-    int my[BQ_KMAX];
-#pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k) my[k] = 0;
-    // subtypes of the owned cells, eight loads in flight
-    for (int i0 = lo; i0 < hi; i0 += 8) {
-        int wb[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) wb[u] = i0 + u < hi ? c.W[i0 + u] : -1;
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-#pragma unroll
-            for (int k = 0; k < BQ_KMAX; ++k) my[k] += (wb[u] == k);
-    }
-    // exclusive prefix of the counts over the threads, per subtype: a scan
-    // within each wave, then the totals of the waves before
     const int lane = tid & 63, wv = tid >> 6;
-    int incl[BQ_KMAX];
+    int base = b0;   // first position of subtype kc's group in the batch's run
+    for (int kc = 0; kc < K; kc += BQ_KREG) {
+        int my[BQ_KREG];
 #pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k) {
-        int v = my[k];
-        if (k < K) {
+        for (int k = 0; k < BQ_KREG; ++k) my[k] = 0;
+        // subtypes of the owned cells, eight loads in flight
+        for (int i0 = lo; i0 < hi; i0 += 8) {
+            int wb[8];
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int o = __shfl_up(v, d);
-                if (lane >= d) v += o;
+            for (int u = 0; u < 8; ++u) wb[u] = i0 + u < hi ? c.W[i0 + u] - kc : -1;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int k = 0; k < BQ_KREG; ++k) my[k] += (wb[u] == k);
+        }
+        // exclusive prefix of the counts over the threads, per subtype: a scan
+        // within each wave, then the totals of the waves before
+        int incl[BQ_KREG];
+#pragma unroll
+        for (int k = 0; k < BQ_KREG; ++k) {
+            int v = my[k];
+            if (kc + k < K) {
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int o = __shfl_up(v, d);
+                    if (lane >= d) v += o;
+                }
+                if (lane == 63) s_wsum[wv][k] = v;
             }
-            if (lane == 63) s_wsum[wv][k] = v;
+            incl[k] = v;
         }
-        incl[k] = v;
-    }
-    __syncthreads();
-    int base = b0;
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < BQ_KMAX; ++k) {
-        if (k < K) {
-            int off = base;
-            for (int w = 0; w < wv; ++w) off += s_wsum[w][k];
-            my[k] = off + incl[k] - my[k];
-            base += s_cnt[b * K + k];
+        for (int k = 0; k < BQ_KREG; ++k) {
+            if (kc + k < K) {
+                int off = base;
+                for (int w = 0; w < wv; ++w) off += s_wsum[w][k];
+                my[k] = off + incl[k] - my[k];
+                base += s_cnt[b * K + kc + k];
+            }
         }
-    }
-    for (int i0 = lo; i0 < hi; i0 += 8) {
-        int wb[8];
+        __syncthreads();   // s_wsum is rewritten by the next chunk
+        for (int i0 = lo; i0 < hi; i0 += 8) {
+            int wb[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) wb[u] = i0 + u < hi ? c.W[i0 + u] : -1;
+            for (int u = 0; u < 8; ++u) wb[u] = i0 + u < hi ? c.W[i0 + u] - kc : -1;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (wb[u] < 0) break;
-            int pos = 0;
+            for (int u = 0; u < 8; ++u) {
+                if (i0 + u >= hi) break;
+                int pos = -1;
 #pragma unroll
-            for (int k = 0; k < BQ_KMAX; ++k)
-                if (wb[u] == k) { pos = my[k]; my[k] += 1; }
-            c.perm[pos] = i0 + u;
+                for (int k = 0; k < BQ_KREG; ++k)
+                    if (wb[u] == k) { pos = my[k]; my[k] += 1; }
+                if (pos >= 0) c.perm[pos] = i0 + u;
+            }
         }
     }
 }
@@ -1863,7 +1946,8 @@ busseq_status validate(const busseq_config* cfg)
             bq_set_error("The sample size in any batch must be greater than the assumed cell type number.");
             return BUSSEQ_ERR_ARG;
         }
-    if (cfg->K < 2 || cfg->K > BQ_KMAX) { bq_set_error("K must be between 2 and 16"); return BUSSEQ_ERR_ARG; }
+    if (cfg->K < 2 || cfg->K > BQ_KMAX) { bq_set_error("K must be between 2 and 64"); return BUSSEQ_ERR_ARG; }
+    if ((long long)cfg->B * cfg->K > BQ_BKMAX) { bq_set_error("B * K must be at most 2048"); return BUSSEQ_ERR_ARG; }
     if (cfg->G < 1 || cfg->G > 65535) { bq_set_error("G must be between 1 and 65535"); return BUSSEQ_ERR_ARG; }
     if (cfg->n_iter < 0 || cfg->n_burnin < 0) { bq_set_error("negative iteration count"); return BUSSEQ_ERR_ARG; }
     if (cfg->precision != 32 && cfg->precision != 64) { bq_set_error("precision must be 32 or 64"); return BUSSEQ_ERR_ARG; }

@@ -90,26 +90,29 @@ __global__ __launch_bounds__(256) void k_init_lists(const int32_t* __restrict__ 
 }
 
 // Block = 256 genes x one chunk of cells of one batch.  Thread g keeps the
-// chunk's K subtype sums of gene g in LDS (its own column: no atomics) and
-// adds the nonzero ones to rs.
+// chunk's subtype sums of gene g in LDS (its own column: no atomics), BQ_KREG
+// subtypes per pass over the chunk's cells, and adds the nonzero ones to rs.
 __global__ __launch_bounds__(256) void k_init_raw(const int32_t* __restrict__ S, int G, int Gp, int K,
                                                   const int32_t* __restrict__ W, const double* __restrict__ ie,
                                                   const int32_t* __restrict__ chunks,
                                                   unsigned long long* __restrict__ rs)
 {
-    __shared__ uint64_t acc[BQ_KMAX][256];
+    __shared__ uint64_t acc[BQ_KREG][256];
     const int t = threadIdx.x;
     const int g = blockIdx.y * 256 + t;
-    for (int k = 0; k < K; ++k) acc[k][t] = 0;
     const int b = chunks[3 * blockIdx.x], st = chunks[3 * blockIdx.x + 1], n = chunks[3 * blockIdx.x + 2];
     if (g >= G) return;
-    for (int r = st; r < st + n; ++r) {
-        const int32_t y = S[(size_t)r * Gp + g];
-        if (y > 0) acc[W[r]][t] += bq_fix(bq_log1p((double)y * ie[r]));
-    }
-    for (int k = 0; k < K; ++k) {
-        const uint64_t v = acc[k][t];
-        if (v) atomicAdd(&rs[((size_t)b * K + k) * G + g], (unsigned long long)v);
+    for (int kc = 0; kc < K; kc += BQ_KREG) {
+        for (int k = 0; k < BQ_KREG; ++k) acc[k][t] = 0;
+        for (int r = st; r < st + n; ++r) {
+            const int32_t y = S[(size_t)r * Gp + g];
+            const int k = W[r] - kc;
+            if (y > 0 && k >= 0 && k < BQ_KREG) acc[k][t] += bq_fix(bq_log1p((double)y * ie[r]));
+        }
+        for (int k = 0; k < BQ_KREG && kc + k < K; ++k) {
+            const uint64_t v = acc[k][t];
+            if (v) atomicAdd(&rs[((size_t)b * K + kc + k) * G + g], (unsigned long long)v);
+        }
     }
 }
 

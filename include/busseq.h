@@ -40,7 +40,7 @@ typedef struct busseq_config {
     int B;                /* batches (-B) */
     const int* n_b;       /* cells per batch, B entries (-N file) */
     int G;                /* genes (-G) */
-    int K;                /* subtypes (-K), 2 <= K <= 16 */
+    int K;                /* subtypes (-K), 2 <= K <= 64, with B * K <= 2048 */
     uint32_t seed;        /* -s */
     int n_iter;           /* -i */
     int n_burnin;         /* -b */

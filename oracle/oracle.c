@@ -28,6 +28,7 @@
 #include "../include/busseq/model.h"
 
 #define H BQ_HIST
+#define ORC_KMAX 64   /* largest K, as the sampler's */
 
 struct orc {
     int B, G, K, N;
@@ -107,7 +108,7 @@ orc_t* orc_create(int B, const int* n_b, int G, int K, const int32_t* Y,
                   uint32_t seed, uint32_t chain, int precision,
                   int n_iter, int n_burnin, int n_unchanged)
 {
-    if (B < 1 || G < 1 || K < 1 || K > 16) return NULL;
+    if (B < 1 || G < 1 || K < 1 || K > ORC_KMAX) return NULL;
     orc_t* o = (orc_t*)zal(sizeof(orc_t));
     o->B = B; o->G = G; o->K = K;
     o->seed = seed; o->chain = chain; o->prec = precision == 32 ? 32 : 64;
@@ -414,10 +415,10 @@ static void pass_w_##SFX(orc_t* o, int t)                                       
     _Pragma("omp parallel for schedule(dynamic, 4)")                                       \
     for (int i = 0; i < o->N; ++i) {                                                       \
         int b = o->batch[i];                                                               \
-        uint64_t acc[16];                                                                  \
+        uint64_t acc[ORC_KMAX];                                                            \
         for (int k = 0; k < K; ++k) acc[k] = 0;                                            \
         for (int c0 = 0; c0 < G; c0 += CHUNK) {                                            \
-            double part[16][LANES];                                                        \
+            double part[ORC_KMAX][LANES];                                                  \
             for (int k = 0; k < K; ++k) for (int l = 0; l < LANES; ++l) part[k][l] = 0.0;  \
             int c1 = c0 + CHUNK < G ? c0 + CHUNK : G;                                      \
             for (int g = c0; g < c1; ++g) {                                                \
@@ -435,7 +436,7 @@ static void pass_w_##SFX(orc_t* o, int t)                                       
             }                                                                              \
             for (int k = 1; k < K; ++k) for (int l = 0; l < LANES; ++l) acc[k] += bq_fix(part[k][l]); \
         }                                                                                  \
-        double lp[16], mx = -BQ_INF;                                                       \
+        double lp[ORC_KMAX], mx = -BQ_INF;                                                 \
         for (int k = 0; k < K; ++k) {                                                      \
             double lk = bq_unfix(acc[k]);                                                  \
             o->w_logit[(size_t)i * K + k] = lk;                                            \
@@ -690,7 +691,7 @@ static void update_pi(orc_t* o, int t)
     for (int i = 0; i < o->N; ++i) o->cnt[o->batch[i] * K + o->W[i]] += 1;
     for (int b = 0; b < B; ++b) {
         bq_rng r; bq_rng_init(&r, o->seed, o->chain, (uint32_t)b, 0u, (uint32_t)t, BQ_PH_PI);
-        double gk[16], tot = 0.0;
+        double gk[ORC_KMAX], tot = 0.0;
         const double xi = g_mut == 1 ? BQ_XI_PI - 1.0 : BQ_XI_PI;   /* 1: planted prior */
         for (int k = 0; k < K; ++k) { gk[k] = bq_rgamma(&r, o->cnt[b * K + k] + xi, 1.0); tot += gk[k]; }
         for (int k = 0; k < K; ++k) o->pi[b * K + k] = gk[k] / tot;
@@ -1019,7 +1020,7 @@ double orc_bic(const orc_t* o, const orc_post* p)
     #pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < N; ++i) {
         int b = o->batch[i];
-        double ll[16], mx = -BQ_INF;
+        double ll[ORC_KMAX], mx = -BQ_INF;
         for (int k = 0; k < K; ++k) {
             uint64_t acc = 0;
             for (int c0 = 0; c0 < G; c0 += CHUNK) {

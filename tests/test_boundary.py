@@ -256,6 +256,18 @@ def test_library_rejects_a_burn_in_as_long_as_the_run():
     assert e.value.status == 1 and "Burn-in iterations must be less" in str(e.value)
 
 
+def test_library_limits_on_k_and_b_times_k():
+    # validation precedes any device call, so these run without a GPU
+    Y = np.ones((140, 4), np.int32)
+    with pytest.raises(bq.BusseqError) as e:
+        bq.Sampler(Y, [70, 70], 65, n_iter=8, n_burnin=4, n_unchanged=2)
+    assert e.value.status == 1 and "K must be between 2 and 64" in str(e.value)
+    Y = np.ones((33 * 64, 4), np.int32)
+    with pytest.raises(bq.BusseqError) as e:
+        bq.Sampler(Y, [64] * 33, 64, n_iter=8, n_burnin=4, n_unchanged=2)
+    assert e.value.status == 1 and "B * K must be at most 2048" in str(e.value)
+
+
 def test_cli_unknown_option_prints_usage():
     p = cli("-z")
     assert p.returncode == 1

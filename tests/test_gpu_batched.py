@@ -50,6 +50,27 @@ def test_heterogeneous_batch_equals_plain_runs(prec):
     sb.close()
 
 
+def test_batch_of_wide_and_narrow_k_equals_plain_runs():
+    # chains with K > 16 (grouped subtype draw) beside chains with K <= 16 in one batch
+    specs = [([26, 24], 90, 20, 21), ([15, 15], 60, 4, 22), ([40, 38], 70, 37, 23)]
+    probs = []
+    for n_b, G, K, ds in specs:
+        Y = bq.simulate(n_b, G, K, ds)
+        probs.append(dict(Y=Y, n_b=n_b, K=K, seed=2000 + ds, chains=2, n_iter=20, n_burnin=8, n_unchanged=3))
+    print(f"[batched] K = {[p['K'] for p in probs]} x 2 chains", flush=True)
+    sb = bq.Sampler.batch(probs)
+    sb.sweeps(12)
+    j = 0
+    for p in probs:
+        for c in range(p["chains"]):
+            kw = {k: v for k, v in p.items() if k not in ("Y", "n_b", "K", "chains")}
+            with bq.Sampler(p["Y"], p["n_b"], p["K"], chains=1, chain0=c, **kw) as sp:
+                sp.sweeps(12)
+                same(sb.state(j), sp.state(), f"chain {j} (K={p['K']})")
+            j += 1
+    sb.close()
+
+
 def test_k_sweep_bic_per_k():
     n_b = [40, 30, 30]
     Y = bq.simulate(n_b, 100, 4, 77)

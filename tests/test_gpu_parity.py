@@ -174,7 +174,11 @@ def test_graph_equals_in_order_launches(small_data):
     ([30, 24, 20, 16], 130, 7, 32),                          # the subtype count of N=20000, G=3000, K=7
     ([30, 24, 20, 16], 130, 7, 64),
     ([26, 22, 20, 18, 16, 15, 14, 13], 200, 12, 32),         # B=8, K=12 as at N=150000, G=5000
-    ([20, 17, 17], 70, 16, 32),                              # the largest K the sampler takes
+    ([20, 17, 17], 70, 16, 32),                              # the largest K with a kernel specialised on K
+    ([30, 28, 26], 150, 20, 32),                             # K > 16: subtype groups of 16 and 3
+    ([40, 36], 120, 33, 32),                                 # two full groups of 16
+    ([50, 46], 100, 40, 64),                                 # three groups, FP64
+    ([70, 66], 64, 64, 32),                                  # the largest K: four groups, lane k = subtype k
 ])
 def test_large_k_instantiations_match_oracle(n_b, G, K, prec):
     Y = bq.simulate(n_b, G, K, 40 + K)

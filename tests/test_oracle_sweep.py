@@ -257,3 +257,26 @@ def test_recovers_synthetic_truth():
     # imputed true counts at the dropout positions track the hidden truth
     zeros = Y == 0
     assert abs(s["X"][zeros].mean() - X[zeros].mean()) < 0.25 * X[zeros].mean()
+
+
+@pytest.fixture(scope="module")
+def wide_sweep():
+    # more subtypes than one register group of the subtype draw (16) holds
+    n_b = [45, 44]
+    G, K = 40, 40
+    Y = bq.simulate(n_b, G, K, 98)
+    o = po.Oracle(Y, n_b, K, seed=SEED, precision=64, n_iter=20, n_burnin=10, n_unchanged=2)
+    o.sweeps(3)
+    t = o.iteration
+    pre = o.state()
+    o.sweeps(1)
+    post = o.state()
+    dbg = o.debug()
+    return dict(prec=64, Y=Y, n_b=n_b, G=G, K=K, t=t, pre=pre, post=post, d=dbg,
+                batch=np.repeat(np.arange(len(n_b)), n_b))
+
+
+def test_subtype_logits_draw_and_pi_with_forty_subtypes(wide_sweep):
+    test_subtype_logits_and_draw(wide_sweep)
+    test_pi_is_a_distribution(wide_sweep)
+    assert wide_sweep["post"]["W"].max() < 40 and len(np.unique(wide_sweep["post"]["W"])) > 16
