@@ -162,8 +162,9 @@ busseq_status busseq_simulate(int B, const int* n_b, int G, int K, uint32_t data
                               int32_t* X_true);
 
 /* Tile autotuner (no counterpart in the reference): times `sweeps` sweeps of
- * the given problem(s) with each tile size in {16, 32, 64, 128} cells and
- * returns the fastest in *tile_rows (and its sweeps/s).  The tile size sets
+ * the given problem(s) with each tile size in {16, 32, 64, 128} cells, twice in
+ * turn, and returns the size with the fastest run in *tile_rows (and its
+ * sweeps/s).  The tile size sets
  * the grouping of the per-gene sums (their last-bit rounding) and not the
  * model; a run is reproducible given the tile_rows it used.  The problems are
  * not modified; the timing chains are discarded. */
