@@ -290,9 +290,9 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
     for (int k = 0; k < KT; ++k) acc[k] = 0;
     for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
         const unsigned c1 = (unsigned)(c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G);
-        double part[KT];
+        T part[KT];   // the lane's terms of the chunk, summed in precision T
 #pragma unroll
-        for (int k = 0; k < KT; ++k) part[k] = 0.0;
+        for (int k = 0; k < KT; ++k) part[k] = (T)0;
         for (unsigned g = (unsigned)(c0 + lane); g < c1; g += 64) {
             const unsigned o = g * (unsigned)sizeof(T);
             int32_t s = *(const BQ_GLOBAL int32_t*)(row + g * 4u);
@@ -306,10 +306,10 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
             T mu0 = *(const BQ_GLOBAL T*)(t0 + o) * edi;   // tmu's last product
             T r0 = tratio(mu0, phi);
 #pragma unroll
-            for (int k = 1; k < KT; ++k) part[k] += (double)ttw(x, phi, r0, v[k].x, v[k].y);
+            for (int k = 1; k < KT; ++k) part[k] += ttw(x, phi, r0, v[k].x, v[k].y);
         }
 #pragma unroll
-        for (int k = 1; k < KT; ++k) acc[k] += bq_fix(part[k]);
+        for (int k = 1; k < KT; ++k) acc[k] += bq_fix((double)part[k]);
     }
     // Subtype k's logit, log pi and exponential are evaluated once, in lane k,
     // instead of in every lane; the maximum, the total and the search then run
@@ -375,9 +375,9 @@ __global__ __launch_bounds__(256) void k_w_wide(const ChainDev* __restrict__ cds
         for (int j = 0; j < BQ_KREG; ++j) acc[j] = 0;
         for (int c0 = 0; c0 < G; c0 += BQ_CHUNK) {
             const unsigned c1 = (unsigned)(c0 + BQ_CHUNK < G ? c0 + BQ_CHUNK : G);
-            double part[BQ_KREG];
+            T part[BQ_KREG];
 #pragma unroll
-            for (int j = 0; j < BQ_KREG; ++j) part[j] = 0.0;
+            for (int j = 0; j < BQ_KREG; ++j) part[j] = (T)0;
             for (unsigned g = (unsigned)(c0 + lane); g < c1; g += 64) {
                 const unsigned o = g * (unsigned)sizeof(T);
                 int32_t s = *(const BQ_GLOBAL int32_t*)(row + g * 4u);
@@ -389,11 +389,11 @@ __global__ __launch_bounds__(256) void k_w_wide(const ChainDev* __restrict__ cds
                 T r0 = tratio(mu0, phi);
 #pragma unroll
                 for (int j = 0; j < BQ_KREG; ++j)
-                    if (j < nk) part[j] += (double)ttw(x, phi, r0, bg[2 * j], bg[2 * j + 1]);
+                    if (j < nk) part[j] += ttw(x, phi, r0, bg[2 * j], bg[2 * j + 1]);
             }
 #pragma unroll
             for (int j = 0; j < BQ_KREG; ++j)
-                if (j < nk) acc[j] += bq_fix(part[j]);
+                if (j < nk) acc[j] += bq_fix((double)part[j]);
         }
 #pragma unroll
         for (int j = 0; j < BQ_KREG; ++j) {

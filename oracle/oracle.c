@@ -418,8 +418,8 @@ static void pass_w_##SFX(orc_t* o, int t)                                       
         uint64_t acc[ORC_KMAX];                                                            \
         for (int k = 0; k < K; ++k) acc[k] = 0;                                            \
         for (int c0 = 0; c0 < G; c0 += CHUNK) {                                            \
-            double part[ORC_KMAX][LANES];                                                  \
-            for (int k = 0; k < K; ++k) for (int l = 0; l < LANES; ++l) part[k][l] = 0.0;  \
+            T part[ORC_KMAX][LANES];                                                       \
+            for (int k = 0; k < K; ++k) for (int l = 0; l < LANES; ++l) part[k][l] = (T)0; \
             int c1 = c0 + CHUNK < G ? c0 + CHUNK : G;                                      \
             for (int g = c0; g < c1; ++g) {                                                \
                 int32_t s = o->S[(size_t)i * G + g];                                       \
@@ -431,10 +431,10 @@ static void pass_w_##SFX(orc_t* o, int t)                                       
                 T r0 = bq_ratio_##SFX(mu0, phi);                                           \
                 for (int k = 1; k < K; ++k) {                                              \
                     size_t kg = (size_t)k * G + g;                                         \
-                    part[k][g & (LANES - 1)] += (double)bq_tw_##SFX(x, phi, r0, bt[kg], em[kg]); \
+                    part[k][g & (LANES - 1)] += bq_tw_##SFX(x, phi, r0, bt[kg], em[kg]);   \
                 }                                                                          \
             }                                                                              \
-            for (int k = 1; k < K; ++k) for (int l = 0; l < LANES; ++l) acc[k] += bq_fix(part[k][l]); \
+            for (int k = 1; k < K; ++k) for (int l = 0; l < LANES; ++l) acc[k] += bq_fix((double)part[k][l]); \
         }                                                                                  \
         double lp[ORC_KMAX], mx = -BQ_INF;                                                 \
         for (int k = 0; k < K; ++k) {                                                      \
