@@ -565,7 +565,7 @@ template <typename T>
 struct XzItem {
     uint32_t pg;           // index of the entry in its cell's zero-entry list (low 16 bits), gene << 16
     int32_t s;
-    uint32_t b0, b1, b2;   // unread words of the entry's Philox block after the Z draw
+    uint32_t b0, b1;       // the next two words of the entry's stream after its Z draw, if any
     T mu;                  // the proposal mean as computed, in precision T
     double phi;
 };
@@ -672,20 +672,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             const XzItem<T> it = qw[idx];
             const int g = (int)(it.pg >> 16), ip = z0 + (int)(it.pg & 0xFFFFu);
             const int xo = it.s < 0 ? ~it.s : 0;
-            // the stream as bq_xz_update leaves it after the Z draw (three
-            // buffered words) or, without a Z draw, before its first block
-            bq_rng r;
-            r.k0 = c.seed; r.k1 = c.key; r.c0 = (uint32_t)i; r.c1 = (uint32_t)g; r.c2 = (uint32_t)t;
-            r.c3 = ((uint32_t)BQ_PH_XZ << 20) + (xo == 0 ? 1u : 0u);
-            r.b0 = it.b0; r.b1 = it.b1; r.b2 = it.b2; r.b3 = 0u; r.n = xo == 0 ? 3 : 0;
-            const double u = bq_uniform(&r);
+            // the next two words of the entry's stream as bq_xz_update reads
+            // them after its Z draw (or, without one, its first two): the
+            // inversion's uniform and the acceptance uniform
+            const double u = bq_uniform_of(it.b0);
             const double mu = (double)it.mu;
             if (xo < XZ_SPT && bq_nb_inv_ok(mu, it.phi)) {
                 // the inversion returns x' <= 255: both acceptance factors are table entries
                 const int xp = bq_rnbinom_inv_t(u, mu, it.phi, s_inv);
                 const double eo = own ? s_odds[xo] : odt[xo];
                 const double ep = own ? s_odds[xp] : odt[xp];
-                const int xi = bq_uniform(&r) * ep <= eo ? xp : xo;
+                const int xi = bq_uniform_of(it.b1) * ep <= eo ? xp : xo;
                 const int32_t sn = ~xi;
                 if (sn != it.s) { row[g] = sn; zs[ip] = sn; }
                 n1 += 1;
@@ -724,11 +721,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             const int p = base + lane;
             const bool act = p < z1;
             int z = 0;
-            bq_rng r;
+            // the first block of the entry's stream, for every listed entry
+            // alike (a lane that draws Z and one that does not run the same
+            // instructions); the words after the Z draw go to the queue
+            bq_u32x4 w = {0u, 0u, 0u, 0u};
             if (act) {
-                bq_rng_init(&r, c.seed, c.key, (uint32_t)i, (uint32_t)gc, (uint32_t)t, BQ_PH_XZ);
-                const int xo = sc < 0 ? ~sc : 0;   // X = 0 redraws Z whatever Z was (cu:493-496)
-                z = xo == 0 ? (bq_uniform(&r) > qz) : 1;
+                w = bq_philox4x32_10((uint32_t)i, (uint32_t)gc, (uint32_t)t, (uint32_t)BQ_PH_XZ << 20,
+                                     c.seed, c.key);
+                if (sc >= -1) {   // X = 0 redraws Z whatever Z was (cu:493-496)
+                    z = bq_uniform_of(w.v0) > qz;
+                    w.v0 = w.v1; w.v1 = w.v2;
+                } else {
+                    z = 1;
+                }
             }
             const bool zq = act && z;
             const T mu = mc * edi;   // the mean of tmu: ((e^alpha e^beta) e^nu) e^delta
@@ -739,7 +744,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 XzItem<T> it;
                 it.pg = (uint32_t)(p - z0) | ((uint32_t)gc << 16);
                 it.s = sc;
-                it.b0 = r.b0; it.b1 = r.b1; it.b2 = r.b2;
+                it.b0 = w.v0; it.b1 = w.v1;
                 it.mu = mu;
                 it.phi = ph;
                 qw[hq ? XZ_QCAP - 1 - (nhi + __popcll(mhi & below)) : nlo + __popcll(mlo & below)] = it;

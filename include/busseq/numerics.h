@@ -126,11 +126,12 @@ BQ_FN uint32_t bq_next32(bq_rng* r)
  * 2^-32 cell: ((w + 1/2) 2^-32) is exact in double.  A resolution of 2^-32
  * bounds the bias of every Bernoulli, inversion and accept/reject decision
  * that uses it by 2^-32. */
-BQ_FN double bq_uniform(bq_rng* r)
+BQ_FN double bq_uniform_of(uint32_t w)
 {
-    uint32_t w = bq_next32(r);
     return ((double)w + 0.5) * 2.3283064365386962890625e-10; /* 2^-32 */
 }
+
+BQ_FN double bq_uniform(bq_rng* r) { return bq_uniform_of(bq_next32(r)); }
 
 /* ------------------------------------------------------------------------ */
 /* double precision elementary functions                                     */
