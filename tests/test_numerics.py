@@ -369,3 +369,45 @@ def test_random_walk_log1p_arguments_stay_in_the_branch_free_range():
     # the MH term with r = 1 and the extreme steps is finite
     t = po.terms("tmh", 32, np.array([[5.0, 2.0, 1.0, -2 * step, c[2]], [5.0, 2.0, 1.0, 2 * step, c[3]]]))
     assert np.isfinite(t).all()
+
+
+HOST_PASS_SRC = r"""
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include "busseq/numerics.h"
+__global__ void k_uses_tables(double* o) { o[threadIdx.x] = bq_log(2.0 + threadIdx.x) + bq_exp(0.5) + bq_inv_tab[3]; }
+int main(int argc, char** argv) {
+    for (int j = 1; j < argc; ++j) {
+        double x;
+        sscanf(argv[j], "%la", &x);
+        printf("%a %a %a %a\n", bq_log(x), bq_exp(x - 1.0), bq_log1p(x - 1.0), bq_lgamma(x));
+    }
+    printf("%a\n", bq_inv_tab[3]);
+    return 0;
+}
+"""
+
+
+def test_host_pass_of_a_hip_source_reads_the_constant_tables(tmp_path):
+    # Host code compiled by hipcc (the host side of engine.hip, init.hip) evaluates the
+    # same numerics as the gcc-built oracle: the coefficient and reciprocal tables that
+    # the device reads from constant memory are ordinary initialised arrays there.
+    import subprocess
+    hipcc = "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc is not installed")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    src = tmp_path / "host_pass.hip"
+    src.write_text(HOST_PASS_SRC)
+    exe = tmp_path / "host_pass"
+    subprocess.run([hipcc, "-O2", "--offload-arch=gfx950", "-ffp-contract=off", "-fno-fast-math",
+                    "-I", inc, str(src), "-o", str(exe)], check=True, capture_output=True)
+    xs = np.array([0.3, 1.0, 2.0, 7.25, 123.5, 1e-3, 4096.0])
+    out = subprocess.run([str(exe)] + [float(x).hex() for x in xs], check=True, capture_output=True,
+                         text=True).stdout.split("\n")
+    got = np.array([[float.fromhex(v) for v in line.split()] for line in out[:len(xs)]])
+    assert np.array_equal(got[:, 0], po.math("log", xs))
+    assert np.array_equal(got[:, 1], po.math("exp", xs - 1.0))
+    assert np.array_equal(got[:, 2], po.math("log1p", xs - 1.0))
+    assert np.array_equal(got[:, 3], po.math("lgamma", xs))
+    assert float.fromhex(out[len(xs)]) == 1.0 / 3.0
