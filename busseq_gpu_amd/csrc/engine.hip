@@ -595,8 +595,6 @@ static_assert(BQ_HLO == 64, "k_xz keeps one low histogram bin per lane");
 #define XZ_QCAP 192  // queue entries per wave: two stacks of < 128 entries, fewer than 192 together
 #define XZ_MU_SPLIT 6.0  // proposals with a larger mean go to the high stack
 
-// four waves per SIMD: the registers of two rounds of gathers in flight fit
-// without spills (at five a round's gathers were waited for at its start)
 template <typename T>
 __device__ void draws_block(const ChainG& c, int blk, uint64_t (*s_r3)[4]);
 
@@ -606,8 +604,11 @@ static_assert(sizeof(XzItem<float>) * 4 * XZ_QCAP >= sizeof(int) * PLAN_LDS_INTS
 // Leading blocks: nplan plan blocks (one per batch), then ndraws blocks of the
 // draws ahead of the parameter passes; neither reads what the X/Z blocks
 // write, nor the other way round.  The rest: four cells per block.
+// five waves per SIMD: the proposal step's double-precision polynomials take
+// their coefficients as scalar operands (constant tables), which leaves the
+// registers of two rounds of gathers in flight within five waves' share
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_xz(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_xz(
     const ChainDev* __restrict__ cds, int nplan, int ndraws)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];

@@ -161,37 +161,20 @@ BQ_FN double bq_uniform(bq_rng* r) { return bq_uniform_of(bq_next32(r)); }
 #define BQ_LOG1P_LO (-0.29289321881345248)   /* sqrt(1/2) - 1 */
 #define BQ_LOG1P_HI 0.41421356237309515      /* sqrt(2) - 1   */
 
-/* (log1p(f) - f) / f^2 on [sqrt(1/2)-1, sqrt(2)-1], degree 19, rel err 4e-17:
- * the leading coefficient, then the rest in Horner order */
-#define BQ_LOG1P_Q64_C0 2.63421908043126387633e-2
-#define BQ_LOG1P_Q64_REST(X)                                                      \
-    X(-5.96236671431324208315e-2) X(6.45516604239284724764e-2)                    \
-    X(-5.66719174788978263813e-2) X(5.68275041380634990396e-2)                    \
-    X(-6.20216185875966524721e-2) X(6.68287182705409738004e-2)                    \
-    X(-7.14852763864529533455e-2) X(7.69162297923115106935e-2)                    \
-    X(-8.33298977274350011202e-2) X(9.09092238018320702997e-2)                    \
-    X(-1.00000116843998718141e-1) X(1.1111111090821723657e-1)                     \
-    X(-1.24999997803443355093e-1) X(1.42857142827786720574e-1)                    \
-    X(-1.66666666687299985794e-1) X(2.00000000000323002736e-1)                    \
-    X(-2.4999999999992503219e-1) X(3.33333333333332593185e-1) X(-5.0e-1)
-#define BQ_ITEM(c) c,
-#define BQ_HORNER(c) p = __builtin_fma(p, f, c);
-BQ_TABLE double bq_log1p_q64_c[20] = {BQ_LOG1P_Q64_C0, BQ_LOG1P_Q64_REST(BQ_ITEM)};
-/* The same polynomial evaluated two ways, to the same bits.  From the table:
- * on the device each coefficient is a scalar-register operand of its fused
- * multiply-add.  With literal coefficients (_lit): for code that runs at the
- * scalar-register limit (the X/Z proposal step), where table coefficients
- * would be spilled to vector-register lanes and read back one by one. */
+/* (log1p(f) - f) / f^2 on [sqrt(1/2)-1, sqrt(2)-1], degree 19, rel err 4e-17 */
+BQ_TABLE double bq_log1p_q64_c[20] = {
+    2.63421908043126387633e-2, -5.96236671431324208315e-2, 6.45516604239284724764e-2,
+    -5.66719174788978263813e-2, 5.68275041380634990396e-2, -6.20216185875966524721e-2,
+    6.68287182705409738004e-2, -7.14852763864529533455e-2, 7.69162297923115106935e-2,
+    -8.33298977274350011202e-2, 9.09092238018320702997e-2, -1.00000116843998718141e-1,
+    1.1111111090821723657e-1, -1.24999997803443355093e-1, 1.42857142827786720574e-1,
+    -1.66666666687299985794e-1, 2.00000000000323002736e-1, -2.4999999999992503219e-1,
+    3.33333333333332593185e-1, -5.0e-1,
+};
 BQ_FN double bq_log1p_q64(double f)
 {
     double p = bq_log1p_q64_c[0];
     for (int j = 1; j < 20; ++j) p = __builtin_fma(p, f, bq_log1p_q64_c[j]);
-    return p;
-}
-BQ_FN double bq_log1p_q64_lit(double f)
-{
-    double p = BQ_LOG1P_Q64_C0;
-    BQ_LOG1P_Q64_REST(BQ_HORNER)
     return p;
 }
 
@@ -239,40 +222,28 @@ BQ_FN double bq_log1p(double y)
  * 1 + y = 2^k m with m in [sqrt(1/2), sqrt(2)), k from the rounded sum t = 1 + y,
  * and the reduced argument f = (1 + y) 2^-k - 1 = fma(y, 2^-k, 2^-k - 1) formed
  * from the unrounded y with one rounding (f = y exactly for k = 0) */
-#define BQ_DEFINE_LOG1P_MID(NAME, POLY)                                            \
-BQ_FN double NAME(double y)                                                        \
-{                                                                                  \
-    double t = 1.0 + y;                                                            \
-    uint64_t kb = (bq_dbits(t) - 0x3FE6A09E667F3BCDull) & 0xFFF0000000000000ull; /* k << 52 */ \
-    double s = bq_bitsd(0x3FF0000000000000ull - kb);                               /* 2^-k */  \
-    double f = __builtin_fma(y, s, s - 1.0);                                       \
-    double eds = (double)(int64_t)kb; /* k 2^52 */                                 \
-    double r = __builtin_fma(f * f, POLY(f), f);                                   \
-    return __builtin_fma(eds, BQ_LN2_HI_S52, __builtin_fma(eds, BQ_LN2_LO_S52, r)); \
+BQ_FN double bq_log1p_mid(double y)
+{
+    double t = 1.0 + y;
+    uint64_t kb = (bq_dbits(t) - 0x3FE6A09E667F3BCDull) & 0xFFF0000000000000ull;   /* k << 52 */
+    double s = bq_bitsd(0x3FF0000000000000ull - kb);                                  /* 2^-k */
+    double f = __builtin_fma(y, s, s - 1.0);
+    double eds = (double)(int64_t)kb;   /* k 2^52 */
+    double r = __builtin_fma(f * f, bq_log1p_q64(f), f);
+    return __builtin_fma(eds, BQ_LN2_HI_S52, __builtin_fma(eds, BQ_LN2_LO_S52, r));
 }
-BQ_DEFINE_LOG1P_MID(bq_log1p_mid, bq_log1p_q64)
-BQ_DEFINE_LOG1P_MID(bq_log1p_mid_lit, bq_log1p_q64_lit)
 
-/* (exp(r) - 1 - r) / r^2 on |r| <= ln2/2, degree 11, rel err 6e-19 (as above:
- * from the table, or with literal coefficients) */
-#define BQ_EXP_E64_C0 1.60867819486130656973e-10
-#define BQ_EXP_E64_REST(X)                                                        \
-    X(2.0918368776542248092e-9) X(2.50520706768841624642e-8)                      \
-    X(2.75572626528368004013e-7) X(2.75573192476098404626e-6)                     \
-    X(2.48015873370290303277e-5) X(1.98412698412631517252e-4)                     \
-    X(1.38888888888788540436e-3) X(8.33333333333333321769e-3)                     \
-    X(4.16666666666666782315e-2) X(1.66666666666666657415e-1) X(5.0e-1)
-BQ_TABLE double bq_exp_e64_c[12] = {BQ_EXP_E64_C0, BQ_EXP_E64_REST(BQ_ITEM)};
-BQ_FN double bq_exp_e64(double f)
+/* (exp(r) - 1 - r) / r^2 on |r| <= ln2/2, degree 11, rel err 6e-19 */
+BQ_TABLE double bq_exp_e64_c[12] = {
+    1.60867819486130656973e-10, 2.0918368776542248092e-9, 2.50520706768841624642e-8,
+    2.75572626528368004013e-7, 2.75573192476098404626e-6, 2.48015873370290303277e-5,
+    1.98412698412631517252e-4, 1.38888888888788540436e-3, 8.33333333333333321769e-3,
+    4.16666666666666782315e-2, 1.66666666666666657415e-1, 5.0e-1,
+};
+BQ_FN double bq_exp_e64(double r)
 {
     double p = bq_exp_e64_c[0];
-    for (int j = 1; j < 12; ++j) p = __builtin_fma(p, f, bq_exp_e64_c[j]);
-    return p;
-}
-BQ_FN double bq_exp_e64_lit(double f)
-{
-    double p = BQ_EXP_E64_C0;
-    BQ_EXP_E64_REST(BQ_HORNER)
+    for (int j = 1; j < 12; ++j) p = __builtin_fma(p, r, bq_exp_e64_c[j]);
     return p;
 }
 
@@ -281,24 +252,21 @@ BQ_FN double bq_pow2i(int k) /* 2^k for k in [-1022, 1023] */
     return bq_bitsd((uint64_t)(int64_t)(k + 1023) << 52);
 }
 
-#define BQ_DEFINE_EXP(NAME, POLY)                                                  \
-BQ_FN double NAME(double x)                                                        \
-{                                                                                  \
-    if (x != x) return x;                                                          \
-    if (x > 709.782712893384) return BQ_INF;                                       \
-    if (x < -745.2) return 0.0;                                                    \
-    double kd = __builtin_rint(x * BQ_INV_LN2);                                    \
-    int k = (int)kd;                                                               \
-    double r = (x - kd * BQ_LN2_HI) - kd * BQ_LN2_LO;                              \
-    double p = 1.0 + (r + (r * r) * POLY(r));                                      \
-    if (k > 1000 || k < -1000) {                                                   \
-        int k1 = k / 2;                                                            \
-        return (p * bq_pow2i(k1)) * bq_pow2i(k - k1);                              \
-    }                                                                              \
-    return p * bq_pow2i(k);                                                        \
+BQ_FN double bq_exp(double x)
+{
+    if (x != x) return x;
+    if (x > 709.782712893384) return BQ_INF;
+    if (x < -745.2) return 0.0;
+    double kd = __builtin_rint(x * BQ_INV_LN2);
+    int k = (int)kd;
+    double r = (x - kd * BQ_LN2_HI) - kd * BQ_LN2_LO;
+    double p = 1.0 + (r + (r * r) * bq_exp_e64(r));
+    if (k > 1000 || k < -1000) {
+        int k1 = k / 2;
+        return (p * bq_pow2i(k1)) * bq_pow2i(k - k1);
+    }
+    return p * bq_pow2i(k);
 }
-BQ_DEFINE_EXP(bq_exp, bq_exp_e64)
-BQ_DEFINE_EXP(bq_exp_lit, bq_exp_e64_lit)
 
 BQ_FN double bq_expm1(double x)
 {
@@ -748,10 +716,8 @@ BQ_FN int bq_nb_inv_ok(double mu, double phi) { return mu <= 30.0 && 3.0 * mu <=
 BQ_FN int bq_rnbinom_inv_t(double u, double mu, double phi, const double* inv)
 {
     double q = mu / (phi + mu);
-    /* -q in (-0.85, 0): often below bq_log1p's fast range, so the branch-free
-     * form; literal coefficients, as the X/Z proposal step runs at its
-     * scalar-register limit */
-    double p = bq_exp_lit(phi * bq_log1p_mid_lit(-q));
+    /* -q in (-0.85, 0): often below bq_log1p's fast range, so the branch-free form */
+    double p = bq_exp(phi * bq_log1p_mid(-q));
     double s = p;
     int k = 0;
     double c = q * (phi * inv[1]);
