@@ -266,15 +266,9 @@ __global__ __launch_bounds__(256) void k_prep(const ChainDev* __restrict__ cds)
 // One wave per cell; KT = K at compile time.
 // ---------------------------------------------------------------------------
 template <typename T, int KT>
-__global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
+__device__ __forceinline__ void w_cell(const ChainG& c, int i, int lane)
 {
     typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
-    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
-    const int lane = threadIdx.x & 63;
-    // the wave's cell, made wave-uniform so that every row and parameter base
-    // below is a scalar and each load is base + 32-bit lane offset
-    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (i >= c.N) return;
     const int t = *c.iter;
     const int G = c.G, b = c.batch[i];
     // subtype 0's mean factors (e^alpha e^beta_0) e^nu_b with e^beta_0 = 1: the
@@ -344,6 +338,37 @@ __global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
         c.W[i] = kk;
         gatomic_add(&c.cnt[(size_t)(blockIdx.x % BQ_STRIPES) * c.B * KT + b * KT + kk], 1);
     }
+}
+
+template <typename T, int KT>
+__global__ __launch_bounds__(256) void k_w(const ChainDev* __restrict__ cds)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    // the wave's cell, made wave-uniform so that every row and parameter base
+    // is a scalar and each load is base + 32-bit lane offset
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= c.N) return;
+    w_cell<T, KT>(c, i, threadIdx.x & 63);
+}
+
+template <typename T, int KT, int KMAX>
+__device__ __forceinline__ void w_dispatch(const ChainG& c, int i, int lane)
+{
+    if (c.K == KT) { w_cell<T, KT>(c, i, lane); return; }
+    if constexpr (KT < KMAX) w_dispatch<T, KT + 1, KMAX>(c, i, lane);
+}
+
+// k_w_mixed: the subtype draw of a batch whose chains have different K (2..KMAX)
+// in one launch: each block's chain (blockIdx.z) takes its own K's body, the
+// same as k_w<T, K>.  Launched once per K, chains of a K-sweep would each
+// fill only part of the GPU, one launch after another.
+template <typename T, int KMAX>
+__global__ __launch_bounds__(256) void k_w_mixed(const ChainDev* __restrict__ cds)
+{
+    const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (i >= c.N) return;
+    w_dispatch<T, 2, KMAX>(c, i, threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -1664,10 +1689,16 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     // beside which it may run.  With ev (per-kernel timing) events bracket each.
 #define EV() do { if (ev) HIPCHK(hipEventRecord(ev[q], st)); ++q; } while (0)
     EV();
-    for (const auto& kr : s->krange) {
-        int maxn = 0;
-        for (int c = kr.first; c < kr.first + kr.second; ++c) maxn = s->ch[c].N > maxn ? s->ch[c].N : maxn;
-        launch_w<T>(s->ch[kr.first].K, dim3((maxn + 3) / 4, 1, kr.second), blk, st, cd + kr.first);
+    if (s->krange.size() > 1 && s->maxK <= BQ_KREG) {
+        // chains of several K, all within the specialised bodies: one launch
+        if (s->maxK <= 10) hipLaunchKernelGGL((k_w_mixed<T, 10>), g_rows, blk, 0, st, cd);
+        else hipLaunchKernelGGL((k_w_mixed<T, BQ_KREG>), g_rows, blk, 0, st, cd);
+    } else {
+        for (const auto& kr : s->krange) {
+            int maxn = 0;
+            for (int c = kr.first; c < kr.first + kr.second; ++c) maxn = s->ch[c].N > maxn ? s->ch[c].N : maxn;
+            launch_w<T>(s->ch[kr.first].K, dim3((maxn + 3) / 4, 1, kr.second), blk, st, cd + kr.first);
+        }
     }
     EV(); hipLaunchKernelGGL(k_xz<T>, g_xz, blk, 0, st, cd, s->maxB, s->maxDraws);
     EV(); hipLaunchKernelGGL(k_xz_tail<T>, dim3((s->maxN + TAIL_CELLS - 1) / TAIL_CELLS, 1, C), blk, 0, st, cd);
