@@ -286,7 +286,9 @@ BQ_FN double bq_bic_elem(double a, double bk, double nb, double dl, double g0, d
                + y * logp + phi * log1mp;
     }
     double xm = 3.0 * mu;
-    int xmax = xm < 2.0e9 ? (int)xm : 2000000000;
+    /* a NaN mean (a chain that has left the finite range) sums nothing here
+     * instead of running to the 2e9 cap: the element is NaN either way */
+    int xmax = xm < 2.0e9 ? (int)xm : (xm >= 2.0e9 ? 2000000000 : 0);
     double s = phi * log1mp, c = 0.0;
     for (int x = 1; x < xmax; ++x) {
         c += bq_log((phi + (x - 1)) / (double)x);

@@ -159,3 +159,14 @@ def test_xz_update_preserves_the_conditional_law(mu, phi, g0, g1):
     wrong[:, 1] = nb
     wrong /= wrong.sum()
     assert 0.5 * np.abs(emp - wrong).sum() > 0.05
+
+
+@pytest.mark.timeout(60)
+def test_bic_element_of_a_nan_mean_returns_at_once():
+    # a chain whose parameters left the finite range: the zero-count element sums no
+    # terms (instead of running its loop to the 2e9 cap) and comes out NaN
+    rows = np.array([[np.nan, 0.0, 0.0, 0.0, -1.0, -0.2, 2.0, 0.0],
+                     [np.nan, 0.0, 0.0, 0.0, -1.0, -0.2, 2.0, 3.0],
+                     [1.0, 0.0, 0.0, 0.0, -1.0, -0.2, np.nan, 0.0]])
+    got = po.terms("bic_elem", 64, rows)
+    assert np.isnan(got).all()
