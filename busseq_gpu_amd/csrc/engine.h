@@ -20,6 +20,8 @@
 #define BQ_HLO 64      // histogram bins kept in striped copies (the frequent small values)
 #define BQ_STRIPES 32 // copies of each per-sweep counter, one per group of workgroups, so that
                       // the atomics of thousands of cells do not queue on one cache line
+#define BQ_PHI_HV 128  // counts 1 <= Y < BQ_PHI_HV of each (batch, gene) are kept as a histogram for
+                      // the phi step's lgamma parts; larger counts and imputed X are summed one by one
 
 // Fields of one chain.  Q qualifies the pointee of every device address:
 // empty on the host, the global address space in kernels (so that loads,
@@ -41,6 +43,7 @@
     Q int32_t* zs;  /* [nz]  their current S values, kept equal to S at those entries */ \
     const Q int32_t* big_off;  /* [B+1] static counts Y >= H, per batch */ \
     const Q int32_t* big;  /* [nbig] their values */ \
+    const Q uint32_t* hist_bg;  /* [B*G*BQ_PHI_HV] number of cells with Y = v of each (batch, gene), 1 <= v < BQ_PHI_HV */ \
     /* state */ \
     Q int* iter;  /* [1] global iteration (sweep) index */ \
     Q int32_t* S;  /* [N*Gp] encoded X/Z (see numerics contract) */ \
@@ -123,6 +126,10 @@ int bq_dev_init_rows(const int32_t* Yd, int N, int G, int Gp, const int32_t* bat
 // lists: genes of the zero entries of row i at zrow[i], values >= BQ_HIST at brow[i], in gene order
 int bq_dev_init_lists(const int32_t* S, int N, int G, int Gp, const int32_t* zrow, const int32_t* brow,
                       uint16_t* zcol, int32_t* big, void* stream);
+// per-(batch, gene) histogram hist_bg[(b*G + g)*BQ_PHI_HV + v] += cells with S_ig = v,
+// 1 <= v < BQ_PHI_HV (added to, so it starts cleared)
+int bq_dev_init_hist_bg(const int32_t* S, int N, int G, int Gp, const int32_t* batch, uint32_t* hist_bg,
+                        void* stream);
 // raw-mean sums rs[(b*K + k)*G + g] += fix(log1p(S_ig ie_i)) over the cells of each
 // chunk (b, first cell, count) with W_i = k (rs starts cleared)
 int bq_dev_init_raw(const int32_t* S, int G, int Gp, int K, const int32_t* W, const double* ie,

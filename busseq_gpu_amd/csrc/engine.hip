@@ -69,10 +69,12 @@ __device__ __forceinline__ float ttmh_l(float x, float p, float d, float l) { re
 __device__ __forceinline__ double ttmh_l(double x, double p, double d, double l) { return bq_tmh_l_f64(x, p, d, l); }
 __device__ __forceinline__ float ttmh2(float x, float p, float r, float d, float c, float l) { return bq_tmh2_f32(x, p, r, d, c, l); }
 __device__ __forceinline__ double ttmh2(double x, double p, double r, double d, double c, double l) { return bq_tmh2_f64(x, p, r, d, c, l); }
-__device__ __forceinline__ float ttphi(int xi, float x, float ph, float pp, float mu, float d, float dop, float ip, float lr, float dl)
-{ return bq_tphi_f32(xi, x, ph, pp, mu, d, dop, ip, lr, dl); }
-__device__ __forceinline__ double ttphi(int xi, double x, double ph, double pp, double mu, double d, double dop, double ip, double lr, double dl)
-{ return bq_tphi_f64(xi, x, ph, pp, mu, d, dop, ip, lr, dl); }
+__device__ __forceinline__ float ttphi_nb(float x, float ph, float pp, float mu, float d, float dop, float ip, float lr)
+{ return bq_tphi_nb_f32(x, ph, pp, mu, d, dop, ip, lr); }
+__device__ __forceinline__ double ttphi_nb(double x, double ph, double pp, double mu, double d, double dop, double ip, double lr)
+{ return bq_tphi_nb_f64(x, ph, pp, mu, d, dop, ip, lr); }
+__device__ __forceinline__ float ttphi_lg(float x, float ph, float d, float dl) { return bq_tphi_lg_f32(x, ph, d, dl); }
+__device__ __forceinline__ double ttphi_lg(double x, double ph, double d, double dl) { return bq_tphi_lg_f64(x, ph, d, dl); }
 
 // sections of a chain's exact per-item sums (gsum)
 __host__ __device__ __forceinline__ size_t gs_beta(int G) { return (size_t)G; }
@@ -1387,41 +1389,104 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
 }
 
 // ---------------------------------------------------------------------------
-// k_phi: phi partial sums (cu:789-804) with the new delta.  Its first gy rows
-// of blocks run prep_item for the next sweep (they read and write nothing the
-// phi sums and k_record use).
+// k_phi: phi partial sums (cu:789-804) with the new delta.  Each cell's term is
+// its NB part (bq_tphi_nb, summed per tile) plus, for a count x >= 1, its
+// lgamma part (bq_tphi_lg), which depends on x, phi and phi' only and enters
+// the item's exact fixed-point sum on its own.  Leading blocks: the first gy
+// rows run prep_item for the next sweep (they read and write nothing the phi
+// sums and k_record use); the next gyh rows add the lgamma parts of the counts
+// 1 <= Y < BQ_PHI_HV from the static per-(batch, gene) histogram, one wave per
+// item.  The tile blocks queue the other counts with x >= 1 (X of the Y = 0
+// entries, and Y >= BQ_PHI_HV) and evaluate their lgamma parts 64 at a time.
 // ---------------------------------------------------------------------------
+#define PHI_QCAP 128   // queued counts per wave: fewer than 64 wait, each cell row adds at most 64
+
 template <typename T>
 // double precision at three waves per SIMD: its lgamma difference keeps more
 // values live than five (or four) waves' registers hold without spills
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 3))) void k_phi(
-    const ChainDev* __restrict__ cds, int gy)
+    const ChainDev* __restrict__ cds, int gy, int gyh)
 {
     const ChainG& c = ((const ChainG*)cds)[blockIdx.z];
+    __shared__ uint32_t s_q[4][PHI_QCAP];          // (x << 6) | lane of the queued counts
+    __shared__ T s_par[4][3][64];                  // phi, d, lgamma(phi') - lgamma(phi) of the wave's genes
+    __shared__ unsigned long long s_acc[4][64];    // fixed-point lgamma parts of the wave's genes
     if ((int)blockIdx.y < gy) {
         prep_item<T>(c, (int)(blockIdx.y * gridDim.x + blockIdx.x) * 256 + (int)threadIdx.x, *c.iter + 1, false);
         return;
     }
-    const int tile = blockIdx.y - gy;
-    if (tile >= *c.ntiles || blockIdx.x * 256 >= c.G) return;
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= c.G) return;
     const int G = c.G;
+    const size_t BG = (size_t)c.B * G;
+    const T* pc = (const T*)c.pc;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if ((int)blockIdx.y < gy + gyh) {
+        // item j = b G + g: its histogram's counts v, lanes v and v + 64
+        const size_t j = ((size_t)(blockIdx.y - gy) * gridDim.x + blockIdx.x) * 4 + wv;
+        if (j >= BG) return;
+        const T phg = ((const T*)c.ph)[j], d = pc[BG + j], dlg0 = pc[5 * BG + j];
+        const BQ_GLOBAL uint32_t* h = c.hist_bg + j * BQ_PHI_HV;
+        uint64_t a = 0;
+        for (int v = lane; v < BQ_PHI_HV; v += 64) {
+            const uint32_t n = h[v];
+            if (v > 0 && n > 0u) a += bq_fix((double)ttphi_lg((T)v, phg, d, dlg0)) * (uint64_t)n;
+        }
+        a = wave_sum_u64(a);
+        if (lane == 0) gsum_add(c.gsum + gs_phi(G, c.K, c.B) + j, a);
+        return;
+    }
+    const int tile = blockIdx.y - gy - gyh;
+    // a wave of genes past G leaves; in the last wave of genes, the lanes past G
+    // (live = false) read gene G - 1's parameters and the row's padding, add
+    // nothing, and take part in the queue's drains
+    if (tile >= *c.ntiles || blockIdx.x * 256 + wv * 64 >= G) return;
+    const int g0 = blockIdx.x * 256 + threadIdx.x;
+    const bool live = g0 < G;
+    const int g = live ? g0 : G - 1;
     const int b = c.tiles[4 * tile], k = c.tiles[4 * tile + 1];
     const int st = c.tiles[4 * tile + 2], nr = c.tiles[4 * tile + 3];
-    const size_t bg = (size_t)b * G + g, BG = (size_t)c.B * G;
-    const T* pc = (const T*)c.pc;
+    const size_t bg = (size_t)b * G + g;
     const T eag = ((const T*)c.ea)[g], ebg = ((const T*)c.eb)[(size_t)k * G + g];
     const T eng = ((const T*)c.en)[bg], phg = ((const T*)c.ph)[bg];
     const T fp = pc[bg], d = pc[BG + bg], dop = pc[2 * BG + bg], ip = pc[3 * BG + bg];
     const T lrat = pc[4 * BG + bg], dlg0 = pc[5 * BG + bg];
+    uint32_t* qw = s_q[wv];
+    T (*pw)[64] = s_par[wv];
+    unsigned long long* aw = s_acc[wv];
+    pw[0][lane] = phg; pw[1][lane] = d; pw[2][lane] = dlg0;
+    aw[lane] = 0ull;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int nq = 0;           // wave-uniform queue length
+    uint64_t lgx = 0;     // lgamma parts of counts too large for the queue's 26 bits, added here
+    // the newest `take` (<= 64) queued counts, one per lane
+    auto drain = [&](int take) {
+        wave_lds_sync();
+        if (lane < take) {
+            const uint32_t e = qw[nq - take + lane];
+            const int gl = (int)(e & 63u);
+            const T lg = ttphi_lg((T)(int)(e >> 6), pw[0][gl], pw[1][gl], pw[2][gl]);
+            atomicAdd(&aw[gl], (unsigned long long)bq_fix((double)lg));
+        }
+        nq -= take;
+        wave_lds_sync();
+    };
+    wave_lds_sync();
     double acc = 0.0;
     for_tile_cells<T>(c, st, nr, g, [&](int32_t s, T edi) {
         const int xi = s < 0 ? ~s : s;
         const T mu = tmu(eag, ebg, eng, edi);
-        acc += (double)ttphi(xi, (T)xi, phg, fp, mu, d, dop, ip, lrat, dlg0);
+        acc += (double)ttphi_nb((T)xi, phg, fp, mu, d, dop, ip, lrat);
+        // counts the histogram does not hold: X of a Y = 0 entry (S < 0 with X >= 1), Y >= BQ_PHI_HV
+        const bool other = live && (s < -1 || s >= BQ_PHI_HV);
+        const bool q = other && xi < (1 << 26);
+        if (other && !q) lgx += bq_fix((double)ttphi_lg((T)xi, phg, d, dlg0));
+        const uint64_t m = __ballot(q);
+        if (q) qw[nq + __popcll(m & below)] = ((uint32_t)xi << 6) | (uint32_t)lane;
+        nq += __popcll(m);
+        if (nq >= 64) drain(64);
     });
-    gsum_add(c.gsum + gs_phi(G, c.K, c.B) + bg, bq_fix(acc));
+    if (nq > 0) drain(nq);
+    if (live) gsum_add(c.gsum + gs_phi(G, c.K, c.B) + bg, bq_fix(acc) + (uint64_t)aw[lane] + lgx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1681,6 +1746,10 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     const int gy_gamma = (int)((2 * s->maxB + 1 + g_tile.x - 1) / g_tile.x);
     // rows of k_phi blocks that carry the next sweep's prep items
     const int gy_prep = (int)((nblk(s->maxPrep) + g_tile.x - 1) / g_tile.x);
+    // rows of k_phi blocks that carry the histogram items (b, g) of each chain, four per block
+    size_t max_bg = 0;
+    for (int c = 0; c < C; ++c) max_bg = std::max(max_bg, (size_t)s->ch[c].B * s->ch[c].G);
+    const int gy_hist = (int)((max_bg + 4 * (size_t)g_tile.x - 1) / (4 * (size_t)g_tile.x));
     const dim3 g_rec(nblk(s->maxRec), 1, C);
     int q = 0;
     // One stream, no branches: a second branch of the captured graph costs a
@@ -1707,7 +1776,8 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     EV(); hipLaunchKernelGGL(k_beta_nu<T>, dim3(g_tile.x, g_tile.y + gy_gamma, C), blk, 0, st, cd, gy_gamma);
     EV(); hipLaunchKernelGGL(k_bn_dec<T>, dim3((s->maxG + BN_GENES - 1) / BN_GENES, 1, C), blk, 0, st, cd);
     EV(); hipLaunchKernelGGL(k_delta<T>, g_rows, blk, 0, st, cd);
-    EV(); hipLaunchKernelGGL(k_phi<T>, dim3(g_tile.x, g_tile.y + gy_prep, C), blk, 0, st, cd, gy_prep);
+    EV(); hipLaunchKernelGGL(k_phi<T>, dim3(g_tile.x, g_tile.y + gy_prep + gy_hist, C), blk, 0, st, cd, gy_prep,
+                             gy_hist);
     EV(); hipLaunchKernelGGL(k_record<T>, g_rec, blk, 0, st, cd);
     EV();
 #undef EV
@@ -2000,6 +2070,7 @@ busseq_status validate(const busseq_config* cfg)
 struct ProblemDev {
     int32_t *batch, *boff, *zrow, *big_off, *big;
     int64_t* hist;
+    uint32_t* hist_bg;
     uint16_t* zcol;
 };
 
@@ -2038,6 +2109,7 @@ busseq_status init_problem(busseq_sampler* s, const int32_t* Y, int B, int N, in
     pd.batch = s->dalloc<int32_t>(N); pd.boff = s->dalloc<int32_t>(B + 1);
     pd.hist = s->dalloc<int64_t>((size_t)B * H); pd.zrow = s->dalloc<int32_t>((size_t)N + 1);
     pd.big_off = s->dalloc<int32_t>((size_t)B + 1);
+    pd.hist_bg = s->dalloc<uint32_t>((size_t)B * G * BQ_PHI_HV);
     S0 = s->dalloc<int32_t>((size_t)N * Gp);
     if (tmp.oom || s->oom) { bq_set_error("device allocation failed"); return BUSSEQ_ERR_NOMEM; }
     HIPCHK(hipDeviceSynchronize());   // the clears of dalloc precede the passes on the sampler's stream
@@ -2045,6 +2117,7 @@ busseq_status init_problem(busseq_sampler* s, const int32_t* Y, int B, int N, in
     HIPCHK(h2d(pd.batch, batch));
     HIPCHK(h2d(pd.boff, boff));
     INITCHK(bq_dev_init_rows(Yd, N, G, Gp, pd.batch, S0, nzr, nbr, libd, pd.hist, s->stream));
+    INITCHK(bq_dev_init_hist_bg(S0, N, G, Gp, pd.batch, pd.hist_bg, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     std::vector<int32_t> nzh, nbh;
     HIPCHK(d2h(nzh, nzr, N));
@@ -2244,7 +2317,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.n_pres = P; c.traces = cfg->keep_traces && P > 0 ? 1 : 0;
             c.tr_rows = c.traces ? 2 * BQ_TRACE_BLOCK : 1;
             c.TR = TR; c.T_max = T_max; c.prec = s->prec;
-            c.batch = pd.batch; c.boff = pd.boff; c.hist_s = pd.hist;
+            c.batch = pd.batch; c.boff = pd.boff; c.hist_s = pd.hist; c.hist_bg = pd.hist_bg;
             c.zrow = pd.zrow; c.zcol = pd.zcol; c.big_off = pd.big_off; c.big = pd.big;
             c.iter = s->dalloc<int>(1);
             c.S = k == 0 ? S0 : s->dalloc<int32_t>((size_t)N * Gp);

@@ -116,6 +116,22 @@ __global__ __launch_bounds__(256) void k_init_raw(const int32_t* __restrict__ S,
     }
 }
 
+// Per-(batch, gene) histogram of the counts 1 <= Y < BQ_PHI_HV (S holds Y there):
+// a block of 256 genes over 64 cells; integer atomics, so any order gives the same counts.
+__global__ __launch_bounds__(256) void k_init_hist_bg(const int32_t* __restrict__ S, int N, int G, int Gp,
+                                                      const int32_t* __restrict__ batch, uint32_t* __restrict__ h)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= G) return;
+    for (int i0 = blockIdx.y * 64; i0 < N; i0 += gridDim.y * 64) {
+        const int i1 = i0 + 64 < N ? i0 + 64 : N;
+        for (int i = i0; i < i1; ++i) {
+            const int32_t v = S[(size_t)i * Gp + g];
+            if (v >= 1 && v < BQ_PHI_HV) atomicAdd(&h[((size_t)batch[i] * G + g) * BQ_PHI_HV + v], 1u);
+        }
+    }
+}
+
 } // namespace
 
 int bq_dev_init_rows(const int32_t* Yd, int N, int G, int Gp, const int32_t* batch, int32_t* S,
@@ -143,5 +159,14 @@ int bq_dev_init_raw(const int32_t* S, int G, int Gp, int K, const int32_t* W, co
     if (K < 1 || K > BQ_KMAX) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_init_raw, dim3(n_chunks, (G + 255) / 256), dim3(256), 0, (hipStream_t)stream, S, G, Gp, K,
                        W, ie, chunks, (unsigned long long*)rs);
+    return (int)hipGetLastError();
+}
+
+int bq_dev_init_hist_bg(const int32_t* S, int N, int G, int Gp, const int32_t* batch, uint32_t* hist_bg,
+                        void* stream)
+{
+    const int gy = (N + 63) / 64 < 65535 ? (N + 63) / 64 : 65535;
+    hipLaunchKernelGGL(k_init_hist_bg, dim3((G + 255) / 256, gy), dim3(256), 0, (hipStream_t)stream, S, N, G, Gp,
+                       batch, hist_bg);
     return (int)hipGetLastError();
 }

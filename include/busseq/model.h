@@ -142,45 +142,52 @@ BQ_FN float bq_tw_f32(float x, float phi, float r0, float beta, float emc)
     return __builtin_fmaf(-(phi + x), bq_log1pf_mid(r0 * emc), x * beta);
 }
 
-BQ_FN double bq_tphi_f64(int xi, double x, double phi, double phip, double mu, double d, double dop,
-                         double iphip, double lrat, double dlg0)
+/*
+ * The phi term of one cell (BUSseq_gpu.cu:799-802) in two parts, each summed on
+ * its own:
+ *   bq_tphi_nb: (phi+x)*log1p(d*mu/(phi*(phi'+mu))) - d*log1p(mu/phi') - x*log(phi'/phi),
+ *     for every cell, summed per tile like the other MH terms;
+ *   bq_tphi_lg: [lgamma(phi'+x) - lgamma(phi+x)] - [lgamma(phi') - lgamma(phi)], for a
+ *     count x >= 1.  It depends on x, phi and phi' only, and each count's value enters
+ *     the exact fixed-point sum of its (batch, gene) separately, so the counts equal to
+ *     v may be added as their number times the value for v.
+ * The single-precision lgamma part takes the Stirling difference directly where both
+ * arguments are at least 1 (both reciprocals from one, bq_rcpf) and bq_lgamma_diff32
+ * otherwise; the single-precision NB part takes 1/(phi' + mu) from bq_rcpf.
+ */
+BQ_FN double bq_tphi_nb_f64(double x, double phi, double phip, double mu, double d, double dop, double iphip,
+                            double lrat)
 {
     double l1 = bq_log1p_mid((dop * mu) / (phip + mu));
     double l2 = bq_log1p_mid(mu * iphip);
-    double t = __builtin_fma(-d, l2, __builtin_fma(phi + x, l1, -(x * lrat)));
-    if (xi > 0) t = t + (bq_lgamma_diff64(phi + x, d) - dlg0);
-    return t;
+    return __builtin_fma(-d, l2, __builtin_fma(phi + x, l1, -(x * lrat)));
 }
 
-/* The single-precision phi term: the expression of bq_tphi_gen_f32 with its two
- * reciprocals, 1/(phi' + mu) and the Stirling difference's 1/((phi + x)(phi' + x)),
- * taken from one reciprocal of their product (bq_rcpf; a lane with x = 0, which has no
- * lgamma difference, uses 1 for the second factor); a lane whose lgamma
- * arguments need the shift (phi + x < 1 or phi' + x < 1) takes bq_lgamma_diff32. */
-BQ_FN float bq_tphi_f32(int xi, float x, float phi, float phip, float mu, float d, float dop, float iphip,
-                        float lrat, float dlg0)
+BQ_FN double bq_tphi_lg_f64(double x, double phi, double d, double dlg0)
 {
-    const float a = phip + mu;
-    const float z = phi + x, zp = z + d;
-    const int direct = xi > 0 && z >= 1.0f && zp >= 1.0f;
-    const float zz = direct ? z * zp : 1.0f;
-    const float r = bq_rcpf(a * zz);
-    const float l1 = bq_log1pf((dop * mu) * (zz * r));
+    return bq_lgamma_diff64(phi + x, d) - dlg0;
+}
+
+BQ_FN float bq_tphi_nb_f32(float x, float phi, float phip, float mu, float d, float dop, float iphip, float lrat)
+{
+    const float l1 = bq_log1pf((dop * mu) * bq_rcpf(phip + mu));
     const float l2 = bq_log1pf(mu * iphip);
-    float t = __builtin_fmaf(-d, l2, __builtin_fmaf(z, l1, -(x * lrat)));
-    if (xi > 0) {
-        float dl;
-        if (direct) {
-            const float w = a * r;   /* 1/(z zp) */
-            const float iz = zp * w, izp = z * w;
-            const float core = __builtin_fmaf(d, bq_logf_norm(zp), __builtin_fmaf(z - 0.5f, bq_log1pf(d * iz), -d));
-            dl = core + (bq_stirling_s32(izp) - bq_stirling_s32(iz));
-        } else {
-            dl = bq_lgamma_diff32(z, d);
-        }
-        t = t + (dl - dlg0);
+    return __builtin_fmaf(-d, l2, __builtin_fmaf(phi + x, l1, -(x * lrat)));
+}
+
+BQ_FN float bq_tphi_lg_f32(float x, float phi, float d, float dlg0)
+{
+    const float z = phi + x, zp = z + d;
+    float dl;
+    if (z >= 1.0f && zp >= 1.0f) {
+        const float w = bq_rcpf(z * zp);   /* 1/(z zp) */
+        const float iz = zp * w, izp = z * w;
+        const float core = __builtin_fmaf(d, bq_logf_norm(zp), __builtin_fmaf(z - 0.5f, bq_log1pf(d * iz), -d));
+        dl = core + (bq_stirling_s32(izp) - bq_stirling_s32(iz));
+    } else {
+        dl = bq_lgamma_diff32(z, d);
     }
-    return t;
+    return dl - dlg0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -623,7 +623,8 @@ static void pass_delta_##SFX(orc_t* o, int t)                                   
         if (bq_log(bq_uniform(&ra)) <= lr) { o->delta[i] = dp; ed[i] = (T)bq_exp(dp); }    \
     }                                                                                      \
 }                                                                                          \
-/* phi sums (cu:789-804) with the new delta */                                             \
+/* phi sums (cu:789-804) with the new delta: the NB parts summed per tile, each   \
+ * count's lgamma part added in fixed point on its own */                                  \
 static void pass_phi_##SFX(orc_t* o)                                                       \
 {                                                                                          \
     const int B = o->B, G = o->G;                                                          \
@@ -655,8 +656,12 @@ static void pass_phi_##SFX(orc_t* o)                                            
                 int xi = s < 0 ? ~s : s;                                                   \
                 size_t bg = (size_t)b * G + g;                                             \
                 T mu = bq_mu_##SFX(ea[g], eb[(size_t)k * G + g], en[bg], ed[i]);           \
-                pp[g] += (double)bq_tphi_##SFX(xi, (T)xi, ph[bg], cst[bg], mu, cst[BG + bg], \
-                          cst[2 * BG + bg], cst[3 * BG + bg], cst[4 * BG + bg], cst[5 * BG + bg]); \
+                pp[g] += (double)bq_tphi_nb_##SFX((T)xi, ph[bg], cst[bg], mu, cst[BG + bg], \
+                          cst[2 * BG + bg], cst[3 * BG + bg], cst[4 * BG + bg]);            \
+                /* each count's lgamma part, exactly, on its own */                        \
+                if (xi > 0)                                                                \
+                    ap[bg] += bq_fix((double)bq_tphi_lg_##SFX((T)xi, ph[bg], cst[BG + bg],   \
+                                                              cst[5 * BG + bg]));          \
             }                                                                              \
         }                                                                                  \
         for (int g = 0; g < G; ++g) ap[(size_t)b * G + g] += bq_fix(pp[g]);                \
@@ -1142,13 +1147,15 @@ void orc_terms(int fn, int prec, const double* a, int na, double* out, int n)
         if (fn == 9) { out[j] = bq_lr_beta(v[0], v[1], (int)v[2], v[3]); continue; }
         if (fn == 2) {
             double f = v[1], fp = v[2], d = fp - f;
-            if (prec == 32)
-                res = bq_tphi_f32((int)v[0], (float)v[0], (float)f, (float)fp, (float)v[3], (float)d,
-                                  (float)(d / f), (float)(1.0 / fp), (float)bq_log1p(d / f),
-                                  (float)bq_lgamma_diff64(f, d));
-            else
-                res = bq_tphi_f64((int)v[0], v[0], f, fp, v[3], d, d / f, 1.0 / fp, bq_log1p(d / f),
-                                  bq_lgamma_diff64(f, d));
+            if (prec == 32) {
+                res = (double)bq_tphi_nb_f32((float)v[0], (float)f, (float)fp, (float)v[3], (float)d,
+                                             (float)(d / f), (float)(1.0 / fp), (float)bq_log1p(d / f));
+                if (v[0] > 0.0)
+                    res += (double)bq_tphi_lg_f32((float)v[0], (float)f, (float)d, (float)bq_lgamma_diff64(f, d));
+            } else {
+                res = bq_tphi_nb_f64(v[0], f, fp, v[3], d, d / f, 1.0 / fp, bq_log1p(d / f));
+                if (v[0] > 0.0) res += bq_tphi_lg_f64(v[0], f, d, bq_lgamma_diff64(f, d));
+            }
         } else if (prec == 32) {
             if (fn == 0) res = bq_tmh_f32((float)v[0], (float)v[1], (float)v[2], (float)v[3], (float)v[4]);
             else if (fn == 1) res = bq_tw_f32((float)v[0], (float)v[1], (float)v[2], (float)v[3], bq_tw_clamp_f32((float)v[4]));
