@@ -43,7 +43,10 @@
     Q int32_t* zs;  /* [nz]  their current S values, kept equal to S at those entries */ \
     const Q int32_t* big_off;  /* [B+1] static counts Y >= H, per batch */ \
     const Q int32_t* big;  /* [nbig] their values */ \
-    const Q uint32_t* hist_bg;  /* [B*G*BQ_PHI_HV] number of cells with Y = v of each (batch, gene), 1 <= v < BQ_PHI_HV */ \
+    const Q uint64_t* hlist;  /* [n_hlist] the nonzero entries of the per-(batch, gene) histogram of the counts \
+                               * 1 <= Y < BQ_PHI_HV: (j << 32) | (n << 7) | v for n cells with Y = v in item \
+                               * j = b G + g, the entries of an item together */ \
+    long long n_hlist; \
     /* state */ \
     Q int* iter;  /* [1] global iteration (sweep) index */ \
     Q int32_t* S;  /* [N*Gp] encoded X/Z (see numerics contract) */ \
@@ -130,6 +133,10 @@ int bq_dev_init_lists(const int32_t* S, int N, int G, int Gp, const int32_t* zro
 // 1 <= v < BQ_PHI_HV (added to, so it starts cleared)
 int bq_dev_init_hist_bg(const int32_t* S, int N, int G, int Gp, const int32_t* batch, uint32_t* hist_bg,
                         void* stream);
+// the histogram as a list: cnt[j] = distinct counts of item j, then its entries
+// (j << 32) | (n << 7) | v from list[off[j]] (off: exclusive prefix of cnt)
+int bq_dev_hist_distinct(const uint32_t* hist_bg, long long BG, uint32_t* cnt, void* stream);
+int bq_dev_hist_list(const uint32_t* hist_bg, long long BG, const uint64_t* off, uint64_t* list, void* stream);
 // raw-mean sums rs[(b*K + k)*G + g] += fix(log1p(S_ig ie_i)) over the cells of each
 // chunk (b, first cell, count) with W_i = k (rs starts cleared)
 int bq_dev_init_raw(const int32_t* S, int G, int Gp, int K, const int32_t* W, const double* ie,

@@ -1395,8 +1395,9 @@ __global__ __launch_bounds__(256) void k_delta(const ChainDev* __restrict__ cds)
 // the item's exact fixed-point sum on its own.  Leading blocks: the first gy
 // rows run prep_item for the next sweep (they read and write nothing the phi
 // sums and k_record use); the next gyh rows add the lgamma parts of the counts
-// 1 <= Y < BQ_PHI_HV from the static per-(batch, gene) histogram, one wave per
-// item.  The tile blocks queue the other counts with x >= 1 (X of the Y = 0
+// 1 <= Y < BQ_PHI_HV from the static per-(batch, gene) histogram, n fix(D(v))
+// for each of its nonzero entries (v, n), 64 entries per wave, whatever the
+// items' sizes.  The tile blocks queue the other counts with x >= 1 (X of the Y = 0
 // entries, and Y >= BQ_PHI_HV) and evaluate their lgamma parts 64 at a time.
 // ---------------------------------------------------------------------------
 #define PHI_QCAP 128   // queued counts per wave: fewer than 64 wait, each cell row adds at most 64
@@ -1421,18 +1422,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if ((int)blockIdx.y < gy + gyh) {
-        // item j = b G + g: its histogram's counts v, lanes v and v + 64
-        const size_t j = ((size_t)(blockIdx.y - gy) * gridDim.x + blockIdx.x) * 4 + wv;
-        if (j >= BG) return;
-        const T phg = ((const T*)c.ph)[j], d = pc[BG + j], dlg0 = pc[5 * BG + j];
-        const BQ_GLOBAL uint32_t* h = c.hist_bg + j * BQ_PHI_HV;
+        // 64 consecutive entries (j, v, n) of the static list per wave: n fix(D(v))
+        // each, summed over the wave's run of each item j and added once per run
+        const long long e0 = ((long long)(blockIdx.y - gy) * gridDim.x + blockIdx.x) * 256 + wv * 64;
+        if (e0 >= c.n_hlist) return;   // wave-uniform
+        uint32_t j = 0xFFFFFFFFu;      // past the list: a run of its own that adds nothing
         uint64_t a = 0;
-        for (int v = lane; v < BQ_PHI_HV; v += 64) {
-            const uint32_t n = h[v];
-            if (v > 0 && n > 0u) a += bq_fix((double)ttphi_lg((T)v, phg, d, dlg0)) * (uint64_t)n;
+        if (e0 + lane < c.n_hlist) {
+            const uint64_t w = c.hlist[e0 + lane];
+            j = (uint32_t)(w >> 32);
+            const int v = (int)(w & 127u);
+            const uint32_t n = (uint32_t)(w >> 7) & 0x1FFFFFFu;
+            a = bq_fix((double)ttphi_lg((T)v, ((const T*)c.ph)[j], pc[BG + j], pc[5 * BG + j])) * (uint64_t)n;
         }
-        a = wave_sum_u64(a);
-        if (lane == 0) gsum_add(c.gsum + gs_phi(G, c.K, c.B) + j, a);
+        // inclusive sums within runs of equal j (a run's lanes are contiguous)
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t lo = __shfl_up((uint32_t)a, o, 64), hi = __shfl_up((uint32_t)(a >> 32), o, 64);
+            const uint32_t jo = __shfl_up(j, o, 64);
+            if (lane >= o && jo == j) a += ((uint64_t)hi << 32) | lo;
+        }
+        const uint32_t jn = __shfl_down(j, 1, 64);
+        if (j != 0xFFFFFFFFu && (lane == 63 || jn != j)) gsum_add(c.gsum + gs_phi(G, c.K, c.B) + j, a);
         return;
     }
     const int tile = blockIdx.y - gy - gyh;
@@ -1746,10 +1757,10 @@ busseq_status enqueue_sweep(busseq_sampler* s, hipEvent_t* ev = nullptr)
     const int gy_gamma = (int)((2 * s->maxB + 1 + g_tile.x - 1) / g_tile.x);
     // rows of k_phi blocks that carry the next sweep's prep items
     const int gy_prep = (int)((nblk(s->maxPrep) + g_tile.x - 1) / g_tile.x);
-    // rows of k_phi blocks that carry the histogram items (b, g) of each chain, four per block
-    size_t max_bg = 0;
-    for (int c = 0; c < C; ++c) max_bg = std::max(max_bg, (size_t)s->ch[c].B * s->ch[c].G);
-    const int gy_hist = (int)((max_bg + 4 * (size_t)g_tile.x - 1) / (4 * (size_t)g_tile.x));
+    // rows of k_phi blocks that carry the entries of each chain's histogram list, 256 per block
+    long long max_hl = 0;
+    for (int c = 0; c < C; ++c) max_hl = std::max(max_hl, s->hd[c].n_hlist);
+    const int gy_hist = (int)((max_hl + 256LL * g_tile.x - 1) / (256LL * g_tile.x));
     const dim3 g_rec(nblk(s->maxRec), 1, C);
     int q = 0;
     // One stream, no branches: a second branch of the captured graph costs a
@@ -2070,7 +2081,8 @@ busseq_status validate(const busseq_config* cfg)
 struct ProblemDev {
     int32_t *batch, *boff, *zrow, *big_off, *big;
     int64_t* hist;
-    uint32_t* hist_bg;
+    uint64_t* hlist;
+    long long n_hlist;
     uint16_t* zcol;
 };
 
@@ -2109,16 +2121,34 @@ busseq_status init_problem(busseq_sampler* s, const int32_t* Y, int B, int N, in
     pd.batch = s->dalloc<int32_t>(N); pd.boff = s->dalloc<int32_t>(B + 1);
     pd.hist = s->dalloc<int64_t>((size_t)B * H); pd.zrow = s->dalloc<int32_t>((size_t)N + 1);
     pd.big_off = s->dalloc<int32_t>((size_t)B + 1);
-    pd.hist_bg = s->dalloc<uint32_t>((size_t)B * G * BQ_PHI_HV);
+    uint32_t* hist_bg = tmp.alloc<uint32_t>((size_t)B * G * BQ_PHI_HV);
+    uint32_t* hcnt = tmp.alloc<uint32_t>((size_t)B * G);
     S0 = s->dalloc<int32_t>((size_t)N * Gp);
     if (tmp.oom || s->oom) { bq_set_error("device allocation failed"); return BUSSEQ_ERR_NOMEM; }
+    HIPCHK(hipMemset(hist_bg, 0, (size_t)B * G * BQ_PHI_HV * sizeof(uint32_t)));
     HIPCHK(hipDeviceSynchronize());   // the clears of dalloc precede the passes on the sampler's stream
     HIPCHK(hipMemcpy(Yd, Y, (size_t)N * G * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(h2d(pd.batch, batch));
     HIPCHK(h2d(pd.boff, boff));
     INITCHK(bq_dev_init_rows(Yd, N, G, Gp, pd.batch, S0, nzr, nbr, libd, pd.hist, s->stream));
-    INITCHK(bq_dev_init_hist_bg(S0, N, G, Gp, pd.batch, pd.hist_bg, s->stream));
+    INITCHK(bq_dev_init_hist_bg(S0, N, G, Gp, pd.batch, hist_bg, s->stream));
+    INITCHK(bq_dev_hist_distinct(hist_bg, (long long)B * G, hcnt, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    {
+        // the histogram's nonzero entries as a list, each item's together
+        std::vector<uint32_t> cnt;
+        HIPCHK(d2h(cnt, hcnt, (size_t)B * G));
+        std::vector<uint64_t> off((size_t)B * G);
+        uint64_t tot = 0;
+        for (size_t j = 0; j < off.size(); ++j) { off[j] = tot; tot += cnt[j]; }
+        pd.n_hlist = (long long)tot;
+        pd.hlist = s->dalloc<uint64_t>(tot);
+        uint64_t* offd = tmp.alloc<uint64_t>(off.size());
+        if (tmp.oom || s->oom) { bq_set_error("device allocation failed"); return BUSSEQ_ERR_NOMEM; }
+        HIPCHK(h2d(offd, off));
+        INITCHK(bq_dev_hist_list(hist_bg, (long long)B * G, offd, pd.hlist, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
     std::vector<int32_t> nzh, nbh;
     HIPCHK(d2h(nzh, nzr, N));
     HIPCHK(d2h(nbh, nbr, N));
@@ -2317,7 +2347,7 @@ extern "C" busseq_status busseq_create_batch(const busseq_problem* probs, int n_
             c.n_pres = P; c.traces = cfg->keep_traces && P > 0 ? 1 : 0;
             c.tr_rows = c.traces ? 2 * BQ_TRACE_BLOCK : 1;
             c.TR = TR; c.T_max = T_max; c.prec = s->prec;
-            c.batch = pd.batch; c.boff = pd.boff; c.hist_s = pd.hist; c.hist_bg = pd.hist_bg;
+            c.batch = pd.batch; c.boff = pd.boff; c.hist_s = pd.hist; c.hlist = pd.hlist; c.n_hlist = pd.n_hlist;
             c.zrow = pd.zrow; c.zcol = pd.zcol; c.big_off = pd.big_off; c.big = pd.big;
             c.iter = s->dalloc<int>(1);
             c.S = k == 0 ? S0 : s->dalloc<int32_t>((size_t)N * Gp);

@@ -132,7 +132,48 @@ __global__ __launch_bounds__(256) void k_init_hist_bg(const int32_t* __restrict_
     }
 }
 
+// The static histogram as a list: the number of distinct counts 1 <= v < BQ_PHI_HV
+// of each item j = b G + g (one thread per item), then the items' entries
+// (j << 32) | (n << 7) | v at off[j], v ascending (n, a number of cells, is below 2^25)
+__global__ __launch_bounds__(256) void k_hist_distinct(const uint32_t* __restrict__ h, long long BG,
+                                                       uint32_t* __restrict__ cnt)
+{
+    const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= BG) return;
+    uint32_t m = 0;
+    for (int v = 1; v < BQ_PHI_HV; ++v) m += h[j * BQ_PHI_HV + v] != 0u;
+    cnt[j] = m;
+}
+
+__global__ __launch_bounds__(256) void k_hist_list(const uint32_t* __restrict__ h, long long BG,
+                                                   const uint64_t* __restrict__ off, uint64_t* __restrict__ list)
+{
+    const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= BG) return;
+    uint64_t o = off[j];
+    for (int v = 1; v < BQ_PHI_HV; ++v) {
+        const uint32_t n = h[j * BQ_PHI_HV + v];
+        if (n) list[o++] = ((uint64_t)j << 32) | ((uint64_t)n << 7) | (uint64_t)v;
+    }
+}
+
 } // namespace
+
+int bq_dev_hist_distinct(const uint32_t* hist_bg, long long BG, uint32_t* cnt, void* stream)
+{
+    if (BG <= 0) return (int)hipSuccess;
+    hipLaunchKernelGGL(k_hist_distinct, dim3((unsigned)((BG + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       hist_bg, BG, cnt);
+    return (int)hipGetLastError();
+}
+
+int bq_dev_hist_list(const uint32_t* hist_bg, long long BG, const uint64_t* off, uint64_t* list, void* stream)
+{
+    if (BG <= 0) return (int)hipSuccess;
+    hipLaunchKernelGGL(k_hist_list, dim3((unsigned)((BG + 255) / 256)), dim3(256), 0, (hipStream_t)stream, hist_bg,
+                       BG, off, list);
+    return (int)hipGetLastError();
+}
 
 int bq_dev_init_rows(const int32_t* Yd, int N, int G, int Gp, const int32_t* batch, int32_t* S,
                      int32_t* nz_row, int32_t* nbig_row, int64_t* lib, int64_t* hist, void* stream)
