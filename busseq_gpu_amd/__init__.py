@@ -75,6 +75,8 @@ def _load():
     vp = _C.c_void_p
     lib.busseq_last_error.restype = _C.c_char_p
     lib.busseq_config_default.argtypes = [_C.POINTER(_Config)]
+    lib.busseq_default_tile_rows.argtypes = [_C.c_longlong]
+    lib.busseq_default_tile_rows.restype = _C.c_int
     lib.busseq_create.argtypes = [_C.POINTER(_Config), _PI, _C.POINTER(vp)]
     lib.busseq_sweeps.argtypes = [vp, _C.c_int]
     lib.busseq_time_sweeps.argtypes = [vp, _C.c_int, _C.POINTER(_C.c_double)]
@@ -123,7 +125,7 @@ def _dp(a):
 
 
 def _config(n_b, G, K, seed, n_iter, n_burnin, n_unchanged, keep_traces=False, precision=32,
-            chains=1, chain0=0, tile_rows=64, device=0):
+            chains=1, chain0=0, tile_rows=0, device=0):
     lib = _load()
     cfg = _Config()
     lib.busseq_config_default(_C.byref(cfg))
@@ -135,6 +137,12 @@ def _config(n_b, G, K, seed, n_iter, n_burnin, n_unchanged, keep_traces=False, p
     cfg.keep_traces, cfg.precision = int(bool(keep_traces)), int(precision)
     cfg.chains, cfg.chain0, cfg.tile_rows, cfg.device = int(chains), int(chain0), int(tile_rows), int(device)
     return cfg, nb
+
+
+def default_tile_rows(cells):
+    """Cells per reduction tile that a sampler with tile_rows=0 uses, for the
+    number of cells over all its chains (busseq_default_tile_rows)."""
+    return int(_load().busseq_default_tile_rows(int(cells)))
 
 
 def simulate(n_b, G, K, data_seed, gamma0_range=(-2.0, 0.0), want_truth=False):
@@ -232,13 +240,13 @@ class Sampler:
     (Sampler.batch) several problems advanced together in the same launches."""
 
     def __init__(self, Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500,
-                 keep_traces=False, precision=32, chains=1, chain0=0, tile_rows=64, device=0):
+                 keep_traces=False, precision=32, chains=1, chain0=0, tile_rows=0, device=0):
         self._init([dict(Y=Y, n_b=n_b, K=K, seed=seed, n_iter=n_iter, n_burnin=n_burnin,
                          n_unchanged=n_unchanged, keep_traces=keep_traces, chains=chains, chain0=chain0)],
                    precision, tile_rows, device)
 
     @classmethod
-    def batch(cls, problems, precision=32, tile_rows=64, device=0):
+    def batch(cls, problems, precision=32, tile_rows=0, device=0):
         """problems: list of dicts with Y, n_b, K and optional seed, chains, chain0,
         n_iter, n_burnin, n_unchanged, keep_traces."""
         obj = cls.__new__(cls)
@@ -353,7 +361,7 @@ def autotune_batch(problems, precision=32, sweeps=20, device=0):
     """Fastest tile size (cells per tile) of {16, 32, 64, 128} for these problems in
     one batched sampler (busseq_autotune_batch); returns (tile_rows, sweeps_per_s)."""
     lib = _load()
-    probs, keep, _ = _problem_array(problems, precision, 64, device)
+    probs, keep, _ = _problem_array(problems, precision, 0, device)
     tr, sps = _C.c_int(), _C.c_double()
     _check(lib.busseq_autotune_batch(probs, len(problems), int(sweeps), _C.byref(tr), _C.byref(sps)))
     return tr.value, sps.value
@@ -368,7 +376,7 @@ def autotune(Y, n_b, K, seed=13579, chains=1, precision=32, sweeps=20, device=0,
 
 
 def run(Y, n_b, K, seed=13579, n_iter=4000, n_burnin=2000, n_unchanged=500, keep_traces=False,
-        precision=32, tile_rows=64, device=0, prefix=None):
+        precision=32, tile_rows=0, device=0, prefix=None):
     """Whole run (busseq_run); writes the reference's output files when prefix is given."""
     lib = _load()
     Y = _np.ascontiguousarray(Y, dtype=_np.int32)

@@ -6,7 +6,8 @@
 //
 // Additive long options (defaults reproduce the reference's behaviour):
 //   --precision=32|64   per-element arithmetic (default 32)
-//   --tile-rows=R|auto  cells per reduction tile (default 64; auto: the fastest of
+//   --tile-rows=R|auto  cells per reduction tile (default 16, 32 or 64 by the number
+//                       of cells, busseq_default_tile_rows; auto: the fastest of
 //                       16, 32, 64, 128 timed on this data, printed)
 //   --device=D          HIP device ordinal (default 0)
 //   --count-cache=PATH  binary copy of the -c counts: read instead of the text when
@@ -70,7 +71,7 @@ int main(int argc, char** argv)
     char n_file[4096] = "";
     char cache[4096] = "";
     int n_file_flag = 0, burnin_flag = 1, unchanged_flag = 1, seed_flag = 1;
-    int precision = 32, tile_rows = 64, device = 0, tune = 0;
+    int precision = 32, tile_rows = 0, device = 0, tune = 0;
 
     static struct option longopts[] = {
         {"precision", required_argument, nullptr, 1000},
@@ -94,7 +95,7 @@ int main(int argc, char** argv)
         case 'p': print_preserved = 1; break;
         case 'o': std::snprintf(output_file, sizeof(output_file), "%s", optarg); break;
         case 1000: precision = std::atoi(optarg); break;
-        case 1001: tune = std::strcmp(optarg, "auto") == 0; tile_rows = tune ? 64 : std::atoi(optarg); break;
+        case 1001: tune = std::strcmp(optarg, "auto") == 0; tile_rows = tune ? 0 : std::atoi(optarg); break;
         case 1002: device = std::atoi(optarg); break;
         case 1003: std::snprintf(cache, sizeof(cache), "%s", optarg); break;
         default:

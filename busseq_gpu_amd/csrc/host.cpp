@@ -37,7 +37,7 @@ extern "C" void busseq_config_default(busseq_config* c)
     c->B = 4; c->n_b = nullptr; c->G = 3000; c->K = 5;
     c->seed = 0; c->n_iter = 4000; c->n_burnin = 2000; c->n_unchanged = 500;
     c->keep_traces = 0; c->precision = 32; c->chains = 1; c->chain0 = 0;
-    c->tile_rows = 64; c->device = 0;
+    c->tile_rows = 0; c->device = 0;   /* tile_rows 0: busseq_default_tile_rows */
 }
 
 // ---------------------------------------------------------------------------

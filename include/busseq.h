@@ -49,7 +49,7 @@ typedef struct busseq_config {
     int precision;        /* 32: single-precision per-element terms (default); 64: double */
     int chains;           /* independent chains on the same data in one launch (default 1) */
     int chain0;           /* Philox key word of the first chain (default 0) */
-    int tile_rows;        /* cells per reduction tile, 1..256 (0 or default: 64) */
+    int tile_rows;        /* cells per reduction tile, 1..256 (0, the default: busseq_default_tile_rows) */
     int device;           /* HIP device ordinal (default 0) */
 } busseq_config;
 
@@ -160,6 +160,12 @@ busseq_status busseq_write_counts_binary(const char* path, int N, int G, const i
 busseq_status busseq_simulate(int B, const int* n_b, int G, int K, uint32_t data_seed,
                               double g0_lo, double g0_hi, int32_t* Y, int32_t* W_true,
                               int32_t* X_true);
+
+/* Cells per reduction tile for a configuration with tile_rows = 0 (no
+ * counterpart in the reference): 16 below 2560 cells over all chains of the
+ * sampler, 32 below 65536, else 64.  The tile size sets the grouping of the
+ * per-gene sums (their last-bit rounding), not the model. */
+int busseq_default_tile_rows(long long cells);
 
 /* Tile autotuner (no counterpart in the reference): times `sweeps` sweeps of
  * the given problem(s) with each tile size in {16, 32, 64, 128} cells, twice in

@@ -135,9 +135,15 @@ def terms(fn, prec, args):
     return out
 
 
+def default_tile_rows(cells):
+    """The sampler's tile size for tile_rows = 0 (busseq_default_tile_rows in
+    busseq_gpu_amd/csrc/engine.hip), for one chain of `cells` cells."""
+    return 16 if cells < 2560 else 32 if cells < 65536 else 64
+
+
 class Oracle:
     def __init__(self, Y, n_b, K, seed=13579, chain=0, precision=32, n_iter=4000, n_burnin=2000,
-                 n_unchanged=500, tile_rows=64, threads=None, keep_traces=False):
+                 n_unchanged=500, tile_rows=None, threads=None, keep_traces=False):
         lib = load()
         if threads:
             lib.orc_set_threads(int(threads))
@@ -150,7 +156,7 @@ class Oracle:
         if not h:
             raise ValueError("oracle rejected the configuration")
         self._h, self._lib = h, lib
-        lib.orc_set_tile_rows(h, int(tile_rows))
+        lib.orc_set_tile_rows(h, int(tile_rows) if tile_rows else default_tile_rows(self.N))
         if lib.orc_init(h) != 0:
             raise ValueError("oracle init failed (zero library size)")
         self.n_pres = max(0, n_iter - n_burnin) if keep_traces else 0

@@ -41,6 +41,18 @@ def test_oracle_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_default_tile_size_rule_of_library_and_oracle():
+    # tile_rows = 0 picks the tile size from the number of cells, in the library
+    # (busseq_default_tile_rows) and, for one chain, in the oracle the same way
+    from oracle import pyoracle as po
+    for cells in (1, 50, 1000, 2559, 2560, 20000, 65535, 65536, 150000, 10**7):
+        assert bq.default_tile_rows(cells) == po.default_tile_rows(cells), cells
+    assert [bq.default_tile_rows(n) for n in (2559, 2560, 65535, 65536)] == [16, 32, 32, 64]
+    cfg = bq._Config()
+    _load().busseq_config_default(C.byref(cfg))
+    assert cfg.tile_rows == 0
+
+
 def test_read_dims_reference_fixture():
     # count_data/demo_dim.txt of the reference (-N file)
     assert bq.read_dims(os.path.join(GOLD, "demo_dim.txt"), 4) == [300, 300, 200, 200]
