@@ -21,9 +21,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/busseq.h"
+
+#define BATCH_SLOTS 128   // the largest B (the reference's __constant__ batch arrays, cu:9-10)
 
 static int fail(const char* what)
 {
@@ -58,100 +61,134 @@ static busseq_status read_counts(const char* count_data, const char* cache, int 
     return BUSSEQ_OK;
 }
 
-int main(int argc, char** argv)
-{
-    // hard-coded defaults, BUSseq_gpu.cu:1028-1038
-    int B = 4;
-    std::vector<int> n_b(200, 0);
-    n_b[0] = 300; n_b[1] = 300; n_b[2] = 200; n_b[3] = 200;
-    int G = 3000, K = 5, seed = 0;
-    char count_data[4096] = "count_data/demo_count.txt";
-    int n_iter = 4000, n_burnin = 2000, n_unchanged = 500, print_preserved = 0;
-    char output_file[4096] = "demo_output";
-    char n_file[4096] = "";
-    char cache[4096] = "";
-    int n_file_flag = 0, burnin_flag = 1, unchanged_flag = 1, seed_flag = 1;
-    int precision = 32, tile_rows = 0, device = 0, tune = 0;
+// What one run does: the reference's defaults (BUSseq_gpu.cu:1028-1038), what
+// the command line changed, and the additive long options.
+struct Settings {
+    int batches = 4;
+    std::vector<int> cells = std::vector<int>(BATCH_SLOTS, 0);   // per batch; the first four default below
+    int genes = 3000, subtypes = 5, seed = 0;
+    int iterations = 4000, burn_in = 2000, unchanged = 500;
+    bool preserved = false;
+    std::string counts = "count_data/demo_count.txt", prefix = "demo_output", dims, cache;
+    // which of the options with derived defaults the command line set
+    bool dims_set = false, burn_in_set = false, unchanged_set = false, seed_set = false;
+    int precision = 32, tile_rows = 0, device = 0;
+    bool tune = false;
+    Settings() { cells[0] = 300; cells[1] = 300; cells[2] = 200; cells[3] = 200; }
+};
 
-    static struct option longopts[] = {
-        {"precision", required_argument, nullptr, 1000},
-        {"tile-rows", required_argument, nullptr, 1001},
-        {"device", required_argument, nullptr, 1002},
-        {"count-cache", required_argument, nullptr, 1003},
+static void usage(const char* prog)
+{
+    std::printf("Error Usage: %s [-b batch] [-n sample_size_file] [-g gene] [-k celltype] [-c count_data_file)] "
+                "[-i iterations] [-b burnin] [-u unchanged_iterations] [-p print_preserved] [-o output_prefix] \n",
+                prog);
+    std::exit(1);
+}
+
+// the reference's short options (cu:1048) and the long ones
+static void parse(int argc, char** argv, Settings& o)
+{
+    enum { OPT_PRECISION = 1000, OPT_TILE_ROWS, OPT_DEVICE, OPT_CACHE };
+    static const struct option longopts[] = {
+        {"precision", required_argument, nullptr, OPT_PRECISION},
+        {"tile-rows", required_argument, nullptr, OPT_TILE_ROWS},
+        {"device", required_argument, nullptr, OPT_DEVICE},
+        {"count-cache", required_argument, nullptr, OPT_CACHE},
         {nullptr, 0, nullptr, 0},
     };
-    int opt;
-    while ((opt = getopt_long(argc, argv, "B:N:G:K:s:c:i:b:u:po:", longopts, nullptr)) != -1) {
+    for (int opt; (opt = getopt_long(argc, argv, "B:N:G:K:s:c:i:b:u:po:", longopts, nullptr)) != -1;) {
+        const char* v = optarg;
         switch (opt) {
-        case 'B': B = std::atoi(optarg); break;
-        case 'N': n_file_flag = 1; std::snprintf(n_file, sizeof(n_file), "%s", optarg); break;
-        case 'G': G = std::atoi(optarg); break;
-        case 'K': K = std::atoi(optarg); break;
-        case 's': seed_flag = 0; seed = std::atoi(optarg); break;
-        case 'c': std::snprintf(count_data, sizeof(count_data), "%s", optarg); break;
-        case 'i': n_iter = std::atoi(optarg); break;
-        case 'b': burnin_flag = 0; n_burnin = std::atoi(optarg); break;
-        case 'u': unchanged_flag = 0; n_unchanged = std::atoi(optarg); break;
-        case 'p': print_preserved = 1; break;
-        case 'o': std::snprintf(output_file, sizeof(output_file), "%s", optarg); break;
-        case 1000: precision = std::atoi(optarg); break;
-        case 1001: tune = std::strcmp(optarg, "auto") == 0; tile_rows = tune ? 0 : std::atoi(optarg); break;
-        case 1002: device = std::atoi(optarg); break;
-        case 1003: std::snprintf(cache, sizeof(cache), "%s", optarg); break;
-        default:
-            std::printf("Error Usage: %s [-b batch] [-n sample_size_file] [-g gene] [-k celltype] [-c count_data_file)] "
-                        "[-i iterations] [-b burnin] [-u unchanged_iterations] [-p print_preserved] [-o output_prefix] \n",
-                        argv[0]);
-            std::exit(1);
+        case 'B': o.batches = std::atoi(v); break;
+        case 'N': o.dims = v; o.dims_set = true; break;
+        case 'G': o.genes = std::atoi(v); break;
+        case 'K': o.subtypes = std::atoi(v); break;
+        case 's': o.seed = std::atoi(v); o.seed_set = true; break;
+        case 'c': o.counts = v; break;
+        case 'i': o.iterations = std::atoi(v); break;
+        case 'b': o.burn_in = std::atoi(v); o.burn_in_set = true; break;
+        case 'u': o.unchanged = std::atoi(v); o.unchanged_set = true; break;
+        case 'p': o.preserved = true; break;
+        case 'o': o.prefix = v; break;
+        case OPT_PRECISION: o.precision = std::atoi(v); break;
+        case OPT_TILE_ROWS:
+            o.tune = std::strcmp(v, "auto") == 0;
+            o.tile_rows = o.tune ? 0 : std::atoi(v);
+            break;
+        case OPT_DEVICE: o.device = std::atoi(v); break;
+        case OPT_CACHE: o.cache = v; break;
+        default: usage(argv[0]);
         }
     }
-    // the reference's limit (its __constant__ batch arrays, cu:9-10); B < 2 meets its own check below
-    if (B > 128) { std::printf("Error: -B must be at most 128.\n"); return 1; }
-    // derived defaults apply only when arguments are given (cu:1092-1111)
-    if (argc > 1) {
-        if (n_file_flag && B > 0) {
-            if (busseq_read_dims(n_file, B, n_b.data()) != BUSSEQ_OK) return fail("reading the -N file");
-        }
-        if (burnin_flag) n_burnin = n_iter / 2;
-        if (unchanged_flag) n_unchanged = 0.3 * n_iter > 500 ? (int)(0.3 * n_iter) : 500;
+}
+
+// The values derived from others when the command line holds any argument
+// (cu:1092-1111): the batch sizes from the -N file, burn-in n_iter/2 and the
+// unchanged iterations max(0.3 n_iter, 500) unless given; then the seed from
+// the clock unless given.  Returns 0, or an exit code.
+static int complete(Settings& o, bool any_args)
+{
+    if (any_args) {
+        if (o.dims_set && o.batches > 0 && busseq_read_dims(o.dims.c_str(), o.batches, o.cells.data()) != BUSSEQ_OK)
+            return fail("reading the -N file");
+        if (!o.burn_in_set) o.burn_in = o.iterations / 2;
+        if (!o.unchanged_set) o.unchanged = 0.3 * o.iterations > 500 ? (int)(0.3 * o.iterations) : 500;
     }
-    if (seed_flag) {
+    if (!o.seed_set) {
         struct timeval tv;
         gettimeofday(&tv, nullptr);
-        seed = (int)tv.tv_usec;
+        o.seed = (int)tv.tv_usec;
     }
-    std::printf("Seed is %d\n", seed);
-    // input checks, cu:1120-1134; a burn-in as long as the run is rejected too (the
-    // reference would divide the posterior sums by zero recorded iterations)
-    if (n_burnin >= n_iter) {
-        std::printf("Burn-in iterations must be less than total number of iterations.\n");
-        return -1;
-    }
-    if (B < 2) {
-        std::printf("The batch number must be greater than one.\n");
-        return -1;
-    }
-    for (int b = 0; b < B; ++b)
-        if (n_b[b] < K) {
-            std::printf("The sample size in any batch must be greater than the assumed cell type number.\n");
-            return -1;
-        }
+    return 0;
+}
+
+// The reference's input checks and messages (cu:1120-1134), with exit status
+// -1; a burn-in as long as the run is rejected too (the reference would divide
+// the posterior sums by zero recorded iterations).
+static bool valid(const Settings& o)
+{
+    const char* why = nullptr;
+    if (o.burn_in >= o.iterations) why = "Burn-in iterations must be less than total number of iterations.";
+    else if (o.batches < 2) why = "The batch number must be greater than one.";
+    else
+        for (int b = 0; b < o.batches && !why; ++b)
+            if (o.cells[b] < o.subtypes) why = "The sample size in any batch must be greater than the assumed cell type number.";
+    if (why) std::printf("%s\n", why);
+    return why == nullptr;
+}
+
+static void stage(const char* line)
+{
+    std::printf("%s\n", line);
+    std::fflush(stdout);
+}
+
+int main(int argc, char** argv)
+{
+    Settings o;
+    parse(argc, argv, o);
+    // the reference's limit (its __constant__ batch arrays, cu:9-10); fewer than two
+    // batches meet its own check below
+    if (o.batches > BATCH_SLOTS) { std::printf("Error: -B must be at most 128.\n"); return 1; }
+    if (int rc = complete(o, argc > 1)) return rc;
+    std::printf("Seed is %d\n", o.seed);
+    if (!valid(o)) return -1;
+    const int B = o.batches, G = o.genes;
     int N = 0;
-    for (int b = 0; b < B; ++b) N += n_b[b];
+    for (int b = 0; b < B; ++b) N += o.cells[b];
 
-    std::printf("Start reading file.\n");
-    std::fflush(stdout);
+    stage("Start reading file.");
     std::vector<int32_t> Y((size_t)N * G);
-    if (read_counts(count_data, cache, N, G, Y.data()) != BUSSEQ_OK) return fail("reading the count file");
+    if (read_counts(o.counts.c_str(), o.cache.c_str(), N, G, Y.data()) != BUSSEQ_OK)
+        return fail("reading the count file");
 
-    std::printf("Start initialization.\n");
-    std::fflush(stdout);
+    stage("Start initialization.");
     busseq_config cfg;
     busseq_config_default(&cfg);
-    cfg.B = B; cfg.n_b = n_b.data(); cfg.G = G; cfg.K = K;
-    cfg.seed = (uint32_t)seed; cfg.n_iter = n_iter; cfg.n_burnin = n_burnin; cfg.n_unchanged = n_unchanged;
-    cfg.keep_traces = print_preserved; cfg.precision = precision; cfg.tile_rows = tile_rows; cfg.device = device;
-    if (tune) {
+    cfg.B = B; cfg.n_b = o.cells.data(); cfg.G = G; cfg.K = o.subtypes; cfg.seed = (uint32_t)o.seed;
+    cfg.n_iter = o.iterations; cfg.n_burnin = o.burn_in; cfg.n_unchanged = o.unchanged;
+    cfg.keep_traces = o.preserved; cfg.precision = o.precision; cfg.tile_rows = o.tile_rows; cfg.device = o.device;
+    if (o.tune) {
         double sps = 0.0;
         if (busseq_autotune(&cfg, Y.data(), 20, &cfg.tile_rows, &sps) != BUSSEQ_OK) return fail("tile autotuning");
         std::printf("Tile rows %d (autotuned, %.1f sweeps/s).\n", cfg.tile_rows, sps);
@@ -160,24 +197,26 @@ int main(int argc, char** argv)
     busseq_sampler* s = nullptr;
     if (busseq_create(&cfg, Y.data(), &s) != BUSSEQ_OK) return fail("initialization");
 
-    std::printf("Start MCMC.\n");
-    std::fflush(stdout);
-    if (busseq_sweeps(s, n_burnin) != BUSSEQ_OK) { busseq_destroy(s); return fail("MCMC"); }
-    std::printf("Burnin completed. Start recording iterations.\n");
-    std::fflush(stdout);
-    if (busseq_sweeps(s, n_iter - n_burnin) != BUSSEQ_OK) { busseq_destroy(s); return fail("MCMC"); }
-
-    std::printf("Start posterior inference.\n");
-    std::fflush(stdout);
+    // burn-in, then the recorded sweeps, then the posterior summaries
+    stage("Start MCMC.");
     busseq_result r;
     std::memset(&r, 0, sizeof(r));
-    if (busseq_finish(s, 0, &r) != BUSSEQ_OK) { busseq_destroy(s); return fail("posterior inference"); }
+    const char* failed = nullptr;
+    if (busseq_sweeps(s, o.burn_in) != BUSSEQ_OK) failed = "MCMC";
+    if (!failed) {
+        stage("Burnin completed. Start recording iterations.");
+        if (busseq_sweeps(s, o.iterations - o.burn_in) != BUSSEQ_OK) failed = "MCMC";
+    }
+    if (!failed) {
+        stage("Start posterior inference.");
+        if (busseq_finish(s, 0, &r) != BUSSEQ_OK) failed = "posterior inference";
+    }
     busseq_destroy(s);
+    if (failed) return fail(failed);
 
-    std::printf("Start writing output files.\n");
-    std::fflush(stdout);
-    if (print_preserved) std::printf("Start writing preserved iterations.\n");
-    busseq_status rc = busseq_write_outputs(&r, output_file, print_preserved);
+    stage("Start writing output files.");
+    if (o.preserved) std::printf("Start writing preserved iterations.\n");
+    const busseq_status rc = busseq_write_outputs(&r, o.prefix.c_str(), o.preserved ? 1 : 0);
     busseq_result_free(&r);
     if (rc != BUSSEQ_OK) return fail("writing outputs");
     return 0;
