@@ -78,6 +78,17 @@ def test_large_counts_overflow_histogram():
     s.close()
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_counts_at_the_phi_queue_bound(prec):
+    # the phi step queues counts below 2^26 for their lgamma parts and adds the
+    # larger ones inline; 2^26 - 1, 2^26 and 7e7 take the two paths
+    n_b = [20, 15]
+    Y = bq.simulate(n_b, 40, 3, 23)
+    Y[1, 5], Y[7, 11], Y[12, 3], Y[30, 5] = 2**26 - 1, 2**26, 70_000_000, 2**26
+    s, o = run_pair(Y, n_b, 3, (1, 2, 3), prec)
+    s.close()
+
+
 def test_edge_batch_equals_k_and_zero_gene():
     n_b = [4, 4, 4]
     Y = bq.simulate(n_b, 65, 4, 11)
