@@ -80,11 +80,13 @@ def test_large_counts_overflow_histogram():
 
 @pytest.mark.parametrize("prec", [32, 64])
 def test_counts_at_the_phi_queue_bound(prec):
-    # the phi step queues counts below 2^26 for their lgamma parts and adds the
-    # larger ones inline; 2^26 - 1, 2^26 and 7e7 take the two paths
+    # the phi step takes the lgamma parts of counts 1..127 from the static
+    # histogram's list, queues the others below 2^26 and adds the larger ones
+    # inline: 127 and 128, 2^26 - 1, 2^26 and 7e7 take the three paths
     n_b = [20, 15]
     Y = bq.simulate(n_b, 40, 3, 23)
     Y[1, 5], Y[7, 11], Y[12, 3], Y[30, 5] = 2**26 - 1, 2**26, 70_000_000, 2**26
+    Y[2:9, 7], Y[9:14, 7], Y[21:30, 8] = 127, 128, 127
     s, o = run_pair(Y, n_b, 3, (1, 2, 3), prec)
     s.close()
 
