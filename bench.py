@@ -18,6 +18,12 @@ cpu_baseline leg times the CPU oracle (OpenMP) on a bounded number of sweeps of
 the same workload on this host.  Further legs (each skippable): c2 with FP64
 per-element terms, c1 on the GPU beside the CPU oracle, the c4 K-sweep batch
 and the c5 heterogeneous batch against their plain runs in sequence.
+
+    python bench.py --batch-only c4|c5 --tile-rows TR [--steps K] [--warmup W]
+
+times one batched sampler of c4 or c5 at a fixed tile size and prints its
+kernel times instead of the benchmark line: the run the profiler passes over
+the batched kernels use (one sampler per process, no autotuning).
 """
 import argparse
 import json
@@ -122,16 +128,52 @@ def fp64_leg(bq, Y, cfg, sweeps=100, warmup=5):
             "kernel_ms": {k: round(v, 5) for k, v in kt.items()}}
 
 
+def c4_problems(bq):
+    """c4: K = 3..10 with 4 chains each on the c1 data, as one list of problems."""
+    cfg = CONFIGS["c1"]
+    Y = bq.simulate(cfg["n_b"], cfg["G"], cfg["K"], cfg["data_seed"], gamma0_range=cfg["g0"])
+    return [dict(Y=Y, n_b=cfg["n_b"], K=K, seed=13579, chains=C4_CHAINS, **PHASE) for K in C4_KS]
+
+
+def c5_problems(bq):
+    """c5: the 16 heterogeneous data sets, one chain each."""
+    return [dict(Y=bq.simulate(d["n_b"], d["G"], d["K"], d["seed"]), n_b=d["n_b"], K=d["K"], seed=13579,
+                 **PHASE) for d in c5_datasets()]
+
+
+def batch_only(bq, which, precision, tile_rows, sweeps, warmup, kernel_sweeps):
+    """One batched sampler of c4 or c5 at a fixed tile size, timed like the plain
+    run (graph replays, then in-order launches with events between the kernels).
+    A process that runs this creates one sampler and builds its graphs once,
+    the shape of run the profiler passes over the batched kernels use."""
+    probs = c4_problems(bq) if which == "c4" else c5_problems(bq)
+    calib_ms = bq.copy_probe(1 << 30)
+    log(f"[bench] copy probe 1 GiB: {calib_ms:.3f} ms = {2 * (1 << 30) / calib_ms / 1e6:.0f} GB/s (read+write)")
+    cells = sum(int(p["Y"].size) * p.get("chains", 1) for p in probs)
+    n_chains = sum(p.get("chains", 1) for p in probs)
+    sb = bq.Sampler.batch(probs, precision=precision, tile_rows=tile_rows)
+    sb.sweeps(warmup)
+    ms = sb.time_sweeps(sweeps)
+    kt = sb.kernel_times(kernel_sweeps)
+    sb.close()
+    rate = n_chains * sweeps / (ms / 1e3)
+    log(f"[bench] batch {which}: {n_chains} chains, {sweeps} sweeps in {ms:.2f} ms -> {rate:.1f} chain-sweeps/s")
+    for k, v in kt.items():
+        log(f"[bench]   {k:14s} {v * 1e3:9.1f} us")
+    return {"mode": "batch-only", "workload": which, "chains": n_chains, "tile_rows": tile_rows,
+            "precision": precision, "sweeps": sweeps, "chain_sweeps_per_s": round(rate, 1),
+            "cell_gene_updates_per_s": round(cells * sweeps / (ms / 1e3), 1),
+            "kernel_ms": {k: round(v, 5) for k, v in kt.items()}}
+
+
 def batched_vs_plain(bq, precision, sweeps=60, warmup=5):
     """c4: K-sweep K=3..10 with 4 chains each (32 chains) on the c1 data in one
     sampler (every chain in the same kernel launches) against the same
     32 chains run one after another in the plain form."""
-    cfg = CONFIGS["c1"]
-    n_b, G = cfg["n_b"], cfg["G"]
-    Y = bq.simulate(n_b, G, cfg["K"], cfg["data_seed"], gamma0_range=cfg["g0"])
+    probs = c4_problems(bq)
+    Y, n_b, G = probs[0]["Y"], probs[0]["n_b"], probs[0]["Y"].shape[1]
     Ks = list(C4_KS)
     chains = C4_CHAINS
-    probs = [dict(Y=Y, n_b=n_b, K=K, seed=13579, chains=chains, **PHASE) for K in Ks]
     tr_b, _ = bq.autotune_batch(probs, precision=precision)
     sb = bq.Sampler.batch(probs, precision=precision, tile_rows=tr_b)
     sb.sweeps(warmup)
@@ -161,10 +203,7 @@ def heterogeneous_batch(bq, precision, sweeps=20, warmup=3):
     """c5: the 16 data sets in one batched sampler (every data set in the same launches)
     against the same 16 chains run one after another in the plain form."""
     sets = c5_datasets()
-    probs = []
-    for d in sets:
-        Y = bq.simulate(d["n_b"], d["G"], d["K"], d["seed"])
-        probs.append(dict(Y=Y, n_b=d["n_b"], K=d["K"], seed=13579, **PHASE))
+    probs = c5_problems(bq)
     cells = sum(sum(d["n_b"]) * d["G"] for d in sets)
     tr_b, _ = bq.autotune_batch(probs, precision=precision, sweeps=5)
     sb = bq.Sampler.batch(probs, precision=precision, tile_rows=tr_b)
@@ -204,6 +243,9 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0,
                     help="cells per tile of the plain run (0: the autotuner's choice); "
                          "fixed, it keeps separate profiler passes on the same tiles")
+    ap.add_argument("--batch-only", choices=("c4", "c5"), default=None,
+                    help="time one batched sampler of c4 or c5 at --tile-rows and print its kernel "
+                         "times instead of the benchmark line (for profiler passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -211,6 +253,14 @@ def main():
         sys.exit(2)
 
     import busseq_gpu_amd as bq
+
+    if args.batch_only:
+        if args.tile_rows <= 0:
+            print("bench.py: --batch-only needs --tile-rows", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps(batch_only(bq, args.batch_only, args.precision, args.tile_rows, args.steps,
+                                    args.warmup, args.kernel_sweeps)), flush=True)
+        return
 
     cfg = CONFIGS[args.config]
     n_b, G, K = cfg["n_b"], cfg["G"], cfg["K"]
