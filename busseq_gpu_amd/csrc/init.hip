@@ -116,19 +116,47 @@ __global__ __launch_bounds__(256) void k_init_raw(const int32_t* __restrict__ S,
     }
 }
 
-// Per-(batch, gene) histogram of the counts 1 <= Y < BQ_PHI_HV (S holds Y there):
-// a block of 256 genes over 64 cells; integer atomics, so any order gives the same counts.
+// Per-(batch, gene) histogram of the counts 1 <= Y < BQ_PHI_HV (S holds Y there).
+// A thread takes one gene over a run of BQ_HB_CELLS cells and counts into LDS bins
+// of its own (bins[v * 256 + thread]: the lanes of a wave stay on distinct banks
+// whatever their values), so the counting needs neither atomics nor barriers.  At
+// the end of the run, and where the batch changes inside it, the thread adds its
+// non-zero bins to h with integer atomics, so any order gives the same counts.  A
+// bin holds at most BQ_HB_CELLS < 2^16 cells between two flushes.
+#define BQ_HB_CELLS 1024
+
+__device__ __forceinline__ void hist_flush(uint16_t* my, uint32_t* __restrict__ h, size_t item)
+{
+    for (int v = 1; v < BQ_PHI_HV; ++v) {
+        const uint32_t n = my[v * 256];
+        if (n) {
+            atomicAdd(&h[item * BQ_PHI_HV + v], n);
+            my[v * 256] = 0;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_init_hist_bg(const int32_t* __restrict__ S, int N, int G, int Gp,
                                                       const int32_t* __restrict__ batch, uint32_t* __restrict__ h)
 {
+    __shared__ uint16_t bins[BQ_PHI_HV * 256];
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= G) return;
-    for (int i0 = blockIdx.y * 64; i0 < N; i0 += gridDim.y * 64) {
-        const int i1 = i0 + 64 < N ? i0 + 64 : N;
+    uint16_t* my = bins + threadIdx.x;
+    for (int v = 0; v < BQ_PHI_HV; ++v) my[v * 256] = 0;
+    for (int i0 = blockIdx.y * BQ_HB_CELLS; i0 < N; i0 += gridDim.y * BQ_HB_CELLS) {
+        const int i1 = i0 + BQ_HB_CELLS < N ? i0 + BQ_HB_CELLS : N;
+        int b = batch[i0];
         for (int i = i0; i < i1; ++i) {
+            const int bi = batch[i];
+            if (bi != b) {
+                hist_flush(my, h, (size_t)b * G + g);
+                b = bi;
+            }
             const int32_t v = S[(size_t)i * Gp + g];
-            if (v >= 1 && v < BQ_PHI_HV) atomicAdd(&h[((size_t)batch[i] * G + g) * BQ_PHI_HV + v], 1u);
+            if (v >= 1 && v < BQ_PHI_HV) my[v * 256] += 1;
         }
+        hist_flush(my, h, (size_t)b * G + g);
     }
 }
 
@@ -206,7 +234,8 @@ int bq_dev_init_raw(const int32_t* S, int G, int Gp, int K, const int32_t* W, co
 int bq_dev_init_hist_bg(const int32_t* S, int N, int G, int Gp, const int32_t* batch, uint32_t* hist_bg,
                         void* stream)
 {
-    const int gy = (N + 63) / 64 < 65535 ? (N + 63) / 64 : 65535;
+    const int runs = (N + BQ_HB_CELLS - 1) / BQ_HB_CELLS;
+    const int gy = runs < 65535 ? runs : 65535;
     hipLaunchKernelGGL(k_init_hist_bg, dim3((G + 255) / 256, gy), dim3(256), 0, (hipStream_t)stream, S, N, G, Gp,
                        batch, hist_bg);
     return (int)hipGetLastError();
