@@ -11,7 +11,8 @@ FLOAT_KEYS = ("alpha", "beta", "nu", "delta", "phi", "gamma", "pi", "ptau")
 
 
 def oracle_threads():
-    """OpenMP threads of the oracle: the host's cores, at most 16 (a GPU box's CPU share)."""
+    """OpenMP threads of the oracle: the host's cores, at most 16 (the CPU share of one GPU on a
+    multi-GPU host)."""
     return min(16, os.cpu_count() or 1)
 
 
